@@ -1,0 +1,169 @@
+"""Benchmark: closest-hit throughput of the traversal microbenchmark.
+
+Workload (one GPU): 1,048,576 random triangles (centres U[0,1]^3, log-uniform
+edges in [1e-3, 1e-1]) and 4,194,304 rays (half coherent primary rays of a
+2048x2048 pinhole, half incoherent).  One step = one closest-hit query of all
+rays, rays and outputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+
+Roofline: algorithmic bytes per ray = 24 (ray in) + 16 (prim, t, u, v out) =
+40 B; achieved GB/s = 40 * rays / kernel time, against 8 TB/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_RAY_CLOSEST = 40
+BYTES_PER_RAY_ANY = 25
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "trace_wide_microbench_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
+    """The oracle (reference algorithm restated in C++) on host cores over a
+    bounded sample of the same rays: closest-hit rays/s."""
+    import numpy as np
+
+    import oracle
+    threads = min(max_threads, oracle.num_threads())
+    t0 = time.time()
+    orc = oracle.OracleScene(scene)
+    build_s = time.time() - t0
+    rng = np.random.default_rng(1234)
+    done, spent, chunk = 0, 0.0, 4096
+    while spent < budget_s and done < len(o):
+        sel = rng.choice(len(o), chunk, replace=False)
+        t1 = time.perf_counter()
+        orc.trace(o[sel], d[sel], threads=threads)
+        spent += time.perf_counter() - t1
+        done += chunk
+        if spent < budget_s / 8:
+            chunk *= 2
+    return {"value": done / spent / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{done} rays drawn uniformly from the same 4,194,304-ray workload through the same "
+                      f"1,048,576-triangle scene; reference median-split tree built in {build_s:.1f} s (untimed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=1 << 20)
+    ap.add_argument("--rays", type=int, default=1 << 22)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from voidray_amd import workloads as W
+    from voidray_amd.device import Context
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    scene = W.random_triangle_scene(args.tris)
+    o, d = W.microbench_rays(args.rays)
+    t0 = time.time()
+    acc = scene.build_acceleration(ctx)
+    build_s = time.time() - t0
+    n = len(o)
+    to = torch.from_numpy(o).to(dev)
+    td = torch.from_numpy(d).to(dev)
+    prim = torch.empty(n, dtype=torch.int32, device=dev)
+    tt = torch.empty(n, dtype=torch.float32, device=dev)
+    tu = torch.empty(n, dtype=torch.float32, device=dev)
+    tv = torch.empty(n, dtype=torch.float32, device=dev)
+    thit = torch.empty(n, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def step(lo=0, cnt=n, any_hit=False):
+        if any_hit:
+            acc.trace_device(to.data_ptr() + 12 * lo, td.data_ptr() + 12 * lo, cnt, any_hit=True,
+                             d_hit=thit.data_ptr() + lo, stream=sp)
+        else:
+            acc.trace_device(to.data_ptr() + 12 * lo, td.data_ptr() + 12 * lo, cnt, d_prim=prim.data_ptr() + 4 * lo,
+                             d_t=tt.data_ptr() + 4 * lo, d_u=tu.data_ptr() + 4 * lo, d_v=tv.data_ptr() + 4 * lo,
+                             stream=sp)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        return ev0.elapsed_time(ev1) / steps, wall
+
+    ms, wall = timed(step, args.steps, args.warmup)
+    fallbacks = acc.last_fallbacks()
+    half = n // 2
+    ms_coh, _ = timed(lambda: step(0, half), max(5, args.steps // 2), 1)
+    ms_inc, _ = timed(lambda: step(half, n - half), max(5, args.steps // 2), 1)
+    ms_any, _ = timed(lambda: step(any_hit=True), max(5, args.steps // 2), 1)
+    hits = int((prim != -1).sum().item())
+
+    mrays = n / (ms * 1e-3) / 1e6
+    achieved = BYTES_PER_RAY_CLOSEST * n / (ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
+    out = {
+        "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "traversal microbench", "triangles": args.tris, "rays": n,
+                   "rays_mix": "1/2 coherent 2048^2 pinhole, 1/2 incoherent", "bvh": "SAH 4-wide, 8-bit quantised",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _traffic_from_profiles()},
+        "cpu_baseline": cpu,
+        "extra": {
+            "coherent_mrays_s": round(half / (ms_coh * 1e-3) / 1e6, 2),
+            "incoherent_mrays_s": round((n - half) / (ms_inc * 1e-3) / 1e6, 2),
+            "any_hit_mrays_s": round(n / (ms_any * 1e-3) / 1e6, 2),
+            "any_hit_roofline_frac": round(BYTES_PER_RAY_ANY * n / (ms_any * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "hit_fraction": round(hits / n, 4),
+            "exact_engine_fallback_rays": fallbacks,
+            "scene_build_s": round(build_s, 2),
+            "scene_stats": acc.stats(),
+            "wall_s": round(wall, 3),
+            "gpu_over_cpu": (round(mrays / cpu["value"], 1) if cpu else None),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

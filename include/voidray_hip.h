@@ -1,0 +1,259 @@
+/*
+ * voidray_hip.h — C ABI of the MI355X path-tracing core (libvoidray_hip.so).
+ *
+ * Every entry point is plain C: POD structs, raw pointers, sizes and an int
+ * status (0 = ok, < 0 = error code; the message is in vr_last_error()).  No
+ * call unwinds across the ABI.  Calls are synchronous at return unless the
+ * name ends in _async (then they are ordered on the stream passed in).
+ *
+ * Each declaration names the reference item it replaces (paths relative to
+ * the voidray repository).
+ */
+#ifndef VOIDRAY_HIP_H
+#define VOIDRAY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VR_OK 0
+#define VR_ERR_INVALID (-1)     /* bad argument / scene description        */
+#define VR_ERR_HIP (-2)         /* HIP runtime failure                     */
+#define VR_ERR_OOM (-3)         /* device allocation failed                */
+#define VR_ERR_UNSUPPORTED (-4) /* feature not available in this build     */
+#define VR_ERR_CANCELLED (-5)   /* host cancel flag observed between passes */
+
+typedef struct vr_ctx vr_ctx;
+typedef struct vr_scene vr_scene;
+
+/* ---- scene description ------------------------------------------------ */
+
+/* One triangle mesh in world space.  Mirrors Mesh::from_buffers /
+ * Mesh::from_file (voidray_renderer/src/core/mesh.rs:46-116).  normals/uvs
+ * may be NULL: that is the Vertex::position constructor (mesh.rs:26-33),
+ * i.e. zero normals and zero uvs. */
+typedef struct vr_mesh {
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    const float* positions; /* 3 * num_vertices */
+    const float* normals;   /* 3 * num_vertices or NULL */
+    const float* uvs;       /* 2 * num_vertices or NULL */
+    const uint32_t* indices;/* 3 * num_triangles */
+} vr_mesh;
+
+/* Materials of voidray_common/src/simple.rs:17-58. */
+#define VR_MAT_LAMBERTIAN 0      /* Lambertian::scatter  simple.rs:102-132 */
+#define VR_MAT_METAL 1           /* Metal::scatter       simple.rs:140-160 */
+#define VR_MAT_DIELECTRIC 2      /* Dielectric::scatter  simple.rs:200-232 */
+#define VR_MAT_EMISSION 3        /* Emission::scatter    simple.rs:175-185 */
+#define VR_MAT_LAMBERTIAN_BSDF 4 /* LambertianBSDF via traits.rs:19-41      */
+
+typedef struct vr_material {
+    int32_t kind;
+    float color[3];         /* albedo, or emission colour * strength      */
+    float param;            /* Metal fuzz, Dielectric ir                  */
+    int32_t albedo_texture; /* Lambertian texture albedo, -1 = none       */
+    int32_t normal_texture; /* Lambertian normal map, -1 = none           */
+} vr_material;
+
+#define VR_TEX_RGB32F 0   /* image.to_rgb32f() layout (texture.rs:37)   */
+#define VR_TEX_RGBA8 1    /* 8-bit unorm, converted as x / 255          */
+#define VR_SAMPLE_NEAREST 0
+#define VR_SAMPLE_BILINEAR 1
+
+/* ImageTexture (voidray_renderer/src/core/texture.rs:28-99). */
+typedef struct vr_texture {
+    uint32_t width, height;
+    int32_t format;       /* VR_TEX_* */
+    int32_t sample_type;  /* VR_SAMPLE_* */
+    const void* data;     /* row-major, width*height texels */
+} vr_texture;
+
+#define VR_ENV_NONE 0
+#define VR_ENV_UNIFORM 1 /* UniformEnvironment  environments.rs:19-33 */
+#define VR_ENV_HDRI 2    /* HDRIEnvironment     environments.rs:35-91 */
+
+typedef struct vr_environment {
+    int32_t kind;
+    float color[3];       /* uniform colour */
+    uint32_t width, height;
+    const float* data;    /* HDRI: RGB f32, width*height*3 */
+} vr_environment;
+
+/* Camera (voidray_renderer/src/core/camera.rs:5-54): eye, unit direction,
+ * orthonormalised up, fov along the longer image side, optional depth of
+ * field (aperture, focal point). */
+typedef struct vr_camera {
+    float eye[3];
+    float direction[3];
+    float up[3];
+    float fov;
+    int32_t has_dof;
+    float aperture;
+    float focal_point[3];
+} vr_camera;
+
+/* Object = (surface, material) (core/scene.rs:30-34). */
+typedef struct vr_object {
+    uint32_t surface;
+    uint32_t material;
+} vr_object;
+
+/* Whole scene (core/scene.rs:36-44).  Surface i is mesh surface_mesh[i]; as in
+ * SceneAcceleration::hit (scene.rs:182-185) the material of a hit on surface
+ * i is objects[i].material, so objects must have at least num_surfaces
+ * entries. */
+typedef struct vr_scene_desc {
+    uint32_t num_meshes;
+    const vr_mesh* meshes;
+    uint32_t num_surfaces;
+    const uint32_t* surface_mesh;
+    uint32_t num_objects;
+    const vr_object* objects;
+    uint32_t num_materials;
+    const vr_material* materials;
+    uint32_t num_textures;
+    const vr_texture* textures;
+    vr_environment environment;
+    vr_camera camera;
+    uint32_t build_flags; /* VR_BUILD_* */
+} vr_scene_desc;
+
+#define VR_BUILD_DEFAULT 0
+#define VR_BUILD_NO_ENV_CDF 1u /* skip the environment importance-sampling tables */
+
+typedef struct vr_scene_stats {
+    uint32_t num_triangles;
+    uint32_t num_surfaces;
+    uint32_t num_qnodes;        /* wide quantised nodes */
+    uint32_t num_ref_nodes;     /* flattened reference median-split nodes */
+    uint32_t qbvh_depth;
+    uint32_t max_leaf_size;
+    double build_ref_seconds;   /* median-split trees (ranks + fallback) */
+    double build_sah_seconds;   /* SAH build + collapse + quantisation   */
+    double upload_seconds;
+    uint64_t device_bytes;
+} vr_scene_stats;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces the rayon pool + RenderThread set-up (render/renderer.rs:35-60).
+ * One context per host thread; not re-entrant. */
+int vr_create(int device, vr_ctx** out);
+int vr_destroy(vr_ctx* ctx);
+/* Thread-local message of the last failing call on this thread. */
+const char* vr_last_error(void);
+/* Device stream used by the synchronous entry points (hipStream_t). */
+void* vr_stream(vr_ctx* ctx);
+
+/* Scene::build_acceleration (core/scene.rs:163-179) + Mesh::from_buffers
+ * BVH build (mesh.rs:111-114): flattens the scene, builds the reference
+ * median-split trees (bvh.rs:48-130) and the SAH quantised wide BVH, and
+ * uploads everything to HBM.  All input pointers are borrowed for the call. */
+int vr_scene_upload(vr_ctx* ctx, const vr_scene_desc* desc, vr_scene** out);
+int vr_scene_release(vr_scene* scene);
+int vr_scene_get_stats(const vr_scene* scene, vr_scene_stats* out);
+
+/* ---- ray queries (SceneAcceleration::hit, core/scene.rs:182-185) -------- */
+
+#define VR_TRACE_CLOSEST 0
+#define VR_TRACE_ANY 1
+/* traversal engines */
+#define VR_ENGINE_WIDE 0      /* quantised BVH4 + robust-winner check + exact fallback */
+#define VR_ENGINE_REFERENCE 1 /* flattened reference trees, both children, no pruning  */
+
+typedef struct vr_trace_desc {
+    uint32_t num_rays;
+    float t_min;          /* SceneAcceleration uses 0.00001 (scene.rs:183) */
+    int32_t mode;         /* VR_TRACE_* */
+    int32_t engine;       /* VR_ENGINE_* */
+} vr_trace_desc;
+
+/* Closest-hit: prim_id = global triangle index (surfaces in order, triangles
+ * in face order) or 0xFFFFFFFF on a miss; t, u, v = Möller–Trumbore values of
+ * Triangle::hit (mesh.rs:144-188).  Any-hit: hit[i] = 1 iff the reference
+ * query returns Some, prim_id/t/u/v may be NULL.
+ * Host pointers: origins/directions are 3*n floats (xyz per ray, directions
+ * already normalised as Ray::new does, ray.rs:12-17). */
+int vr_trace(vr_scene* scene, const vr_trace_desc* desc,
+             const float* origins, const float* directions,
+             uint32_t* prim_id, float* t, float* u, float* v, uint8_t* hit);
+/* Same with device pointers, enqueued on `stream` (hipStream_t, NULL = the
+ * context stream).  Any pointer among the outputs may be NULL. */
+int vr_trace_device_async(vr_scene* scene, const vr_trace_desc* desc,
+                          const float* d_origins, const float* d_directions,
+                          uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
+                          uint8_t* d_hit, void* stream);
+/* Rays of the last trace that needed the exact fallback engine. */
+int vr_trace_last_fallbacks(vr_scene* scene, uint32_t* count);
+
+/* ---- rendering (render/iterative.rs:11-55, core/tracer.rs:9-56) ------- */
+
+#define VR_MODE_FULL 0    /* RenderMode::Full   */
+#define VR_MODE_NORMAL 1  /* RenderMode::Normal */
+
+#define VR_RENDER_ROWS_BY_HEIGHT 1u /* reproduce y = i / height (iterative.rs:26) */
+#define VR_RENDER_ENV_IS 2u         /* environment importance sampling + MIS     */
+#define VR_RENDER_RUSSIAN_ROULETTE 4u
+#define VR_RENDER_ALPHA_PER_CALL 8u /* alpha += 1 per call, as iterative.rs:51   */
+
+typedef struct vr_render_desc {
+    uint32_t width, height;
+    uint32_t samples;        /* samples traced by this call (iterative_render `samples`) */
+    uint32_t total_samples;  /* RenderSettings::total_samples (scale 1/total)          */
+    uint32_t sample_offset;  /* index of the first sample of this call in the stream  */
+    uint32_t max_bounces;    /* RenderSettings::max_bounces */
+    float firefly_clamp;     /* RenderSettings::firefly_clamp (inf = off) */
+    int32_t render_mode;     /* VR_MODE_* */
+    uint32_t flags;          /* VR_RENDER_* */
+    uint32_t rr_depth;       /* Russian roulette from this depth (flag RR) */
+    uint64_t seed;           /* Philox4x32-10 key */
+    uint32_t request;        /* request id (third counter word) */
+    uint32_t engine;         /* VR_ENGINE_* for extension rays */
+} vr_render_desc;
+
+typedef struct vr_render_stats {
+    uint64_t segments;        /* rays traced (all bounces) */
+    uint64_t samples;         /* camera samples */
+    uint64_t env_misses;
+    uint64_t fallback_rays;   /* rays re-traced by the exact engine */
+    uint64_t capped_loops;    /* rejection loops that hit their cap */
+    double kernel_ms;         /* device time of the render passes */
+} vr_render_stats;
+
+/* Adds (sum over `samples` of L) * (1/total_samples) into rgba_accum (host,
+ * width*height*4 f32, additive, like iterative.rs:45-51). */
+int vr_render(vr_scene* scene, const vr_render_desc* desc, float* rgba_accum,
+              vr_render_stats* stats);
+/* Same into a device accumulator. */
+int vr_render_device_async(vr_scene* scene, const vr_render_desc* desc,
+                           float* d_rgba_accum, vr_render_stats* stats, void* stream);
+
+/* ---- post-processing (shaders/post_process.glsl:23-49) ----------------- */
+
+#define VR_TONEMAP_NONE 0
+#define VR_TONEMAP_ACES 1
+#define VR_TONEMAP_REINHARD 2
+#define VR_TONEMAP_FILMIC 3
+#define VR_TONEMAP_UNCHARTED2 4
+
+typedef struct vr_post_desc {
+    uint32_t width, height;
+    float scale;     /* total/current samples (voidray_app/src/main.rs:75-79) */
+    float gamma;
+    float exposure;
+    int32_t tonemap; /* VR_TONEMAP_* (Tonemap::as_i32, settings.rs:44-54) */
+    uint32_t clip_to_1024; /* reproduce the fixed 1024x1024 dispatch (post_process.rs:74) */
+} vr_post_desc;
+
+int vr_tonemap(vr_ctx* ctx, const vr_post_desc* desc, const float* src_rgba, float* dst_rgba);
+int vr_tonemap_device_async(vr_ctx* ctx, const vr_post_desc* desc, const float* d_src,
+                            float* d_dst, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOIDRAY_HIP_H */

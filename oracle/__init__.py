@@ -1,0 +1,164 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of voidray's CPU path (oracle/voidray_oracle.cpp) loaded via
+ctypes.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+may import this package; the product library never links or calls it.
+
+Pinning status (see tests/test_oracle.py):
+  * Philox4x32-10                    — Random123 known-answer vectors
+  * slab test / Möller–Trumbore      — closed-form hits, edge and epsilon cases, brute force
+  * median-split tree                — invariants + an independent recursive Python build
+  * textures / HDRI lookups          — hand-built images with known bilinear answers
+  * materials (Schlick, reflect, refract, Marsaglia sphere) — closed forms, moments
+  * integrator (nested clamp)        — special cases: emissive-only min(E, clamp),
+                                       uniform environment with no geometry
+  * tonemaps                         — FP64 evaluations of the published curves
+  * cgmath association order, rand/rand_distr streams, libm transcendentals:
+    parity unpinned (third-party code not available here).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from voidray_amd import _native as N
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+_P = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build
+            build.build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_philox4x32_10.argtypes = [_P, _P, _P]
+        L.orc_path_words.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
+        L.orc_unit_sphere_f32.argtypes = [C.c_uint64, C.c_uint32, _P]
+        for nm in ("orc_scene_new_f32", "orc_scene_new_f64"):
+            getattr(L, nm).restype = _P
+            getattr(L, nm).argtypes = [C.POINTER(N.vr_scene_desc)]
+        L.orc_scene_free_f32.argtypes = [_P]
+        L.orc_scene_free_f64.argtypes = [_P]
+        L.orc_scene_num_prims.restype = C.c_uint32
+        L.orc_scene_num_prims.argtypes = [_P]
+        L.orc_mesh_leaf_order.restype = C.c_uint32
+        L.orc_mesh_leaf_order.argtypes = [_P, C.c_uint32, _P]
+        L.orc_mesh_tree.restype = C.c_uint32
+        L.orc_mesh_tree.argtypes = [_P, C.c_uint32, _P, _P]
+        L.orc_trace_f32.argtypes = [_P, C.c_uint32, _P, _P, C.c_float, C.c_int, _P, _P, _P, _P, _P, C.c_int, _P]
+        L.orc_trace_bruteforce_f32.argtypes = [_P, C.c_uint32, _P, _P, C.c_float, _P, _P, _P, _P, _P, C.c_int]
+        L.orc_render_f32.restype = C.c_double
+        L.orc_render_f32.argtypes = [_P, C.POINTER(N.vr_render_desc), _P, C.c_int, C.c_uint32, _P]
+        L.orc_render_f64.restype = C.c_double
+        L.orc_render_f64.argtypes = [_P, C.POINTER(N.vr_render_desc), _P, C.c_int, C.c_uint32, C.c_int, _P]
+        L.orc_tonemap.argtypes = [C.POINTER(N.vr_post_desc), _P, _P]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def path_words(seed, request, pixel, sample, count):
+    out = np.zeros(count, np.uint32)
+    lib().orc_path_words(seed, request, pixel, sample, count, out.ctypes.data)
+    return out
+
+
+def unit_sphere_samples(seed, count):
+    out = np.zeros((count, 3), np.float32)
+    lib().orc_unit_sphere_f32(seed, count, out.ctypes.data)
+    return out
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+class OracleScene:
+    """The scene restated on the CPU (f32 = reference parity, f64 = convergence oracle)."""
+
+    def __init__(self, scene, precision="f32"):
+        self.precision = precision
+        desc, self._keep = scene.to_desc()
+        f = lib().orc_scene_new_f32 if precision == "f32" else lib().orc_scene_new_f64
+        self.h = f(C.byref(desc))
+        self.scene = scene
+
+    def __del__(self):
+        try:
+            (lib().orc_scene_free_f32 if self.precision == "f32" else lib().orc_scene_free_f64)(self.h)
+        except Exception:
+            pass
+
+    def trace(self, o, d, t_min=1e-5, threads=0):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = len(o)
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        hit = np.zeros(n, np.uint8)
+        cnt = np.zeros(2, np.uint64)
+        lib().orc_trace_f32(self.h, n, o.ctypes.data, d.ctypes.data, t_min, 0, prim.ctypes.data, t.ctypes.data,
+                            u.ctypes.data, v.ctypes.data, hit.ctypes.data, threads, cnt.ctypes.data)
+        return dict(prim=prim, t=t, u=u, v=v, hit=hit, boxes=int(cnt[0]), tris=int(cnt[1]))
+
+    def trace_bruteforce(self, o, d, rank, t_min=1e-5, threads=0):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        rank = np.ascontiguousarray(rank, np.uint32)
+        n = len(o)
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().orc_trace_bruteforce_f32(self.h, n, o.ctypes.data, d.ctypes.data, t_min, rank.ctypes.data,
+                                       prim.ctypes.data, t.ctypes.data, u.ctypes.data, v.ctypes.data, threads)
+        return dict(prim=prim, t=t, u=u, v=v)
+
+    def leaf_order(self, mesh):
+        n = lib().orc_mesh_leaf_order(self.h, mesh, None)
+        out = np.zeros(n, np.uint32)
+        lib().orc_mesh_leaf_order(self.h, mesh, out.ctypes.data)
+        return out
+
+    def mesh_tree(self, mesh):
+        n = lib().orc_mesh_tree(self.h, mesh, None, None)
+        boxes = np.zeros((n, 6), np.float32)
+        links = np.zeros((n, 3), np.int64)
+        lib().orc_mesh_tree(self.h, mesh, boxes.ctypes.data, links.ctypes.data)
+        return boxes, links
+
+    def render(self, desc, rgba=None, threads=0, row_step=1, prune=False):
+        """Returns (rgba, seconds, stats)."""
+        h, w = desc.height, desc.width
+        st = np.zeros(5, np.uint64)
+        if self.precision == "f32":
+            rgba = np.zeros((h, w, 4), np.float32) if rgba is None else rgba
+            sec = lib().orc_render_f32(self.h, C.byref(desc), rgba.ctypes.data, threads, row_step, st.ctypes.data)
+        else:
+            rgba = np.zeros((h, w, 4), np.float64) if rgba is None else rgba
+            sec = lib().orc_render_f64(self.h, C.byref(desc), rgba.ctypes.data, threads, row_step, int(prune),
+                                       st.ctypes.data)
+        stats = dict(segments=int(st[0]), misses=int(st[1]), capped=int(st[2]), boxes=int(st[3]), tris=int(st[4]))
+        return rgba, sec, stats
+
+
+def tonemap(src, post):
+    src = np.ascontiguousarray(src, np.float32)
+    dst = np.zeros_like(src)
+    lib().orc_tonemap(C.byref(post), src.ctypes.data, dst.ctypes.data)
+    return dst

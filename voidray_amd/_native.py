@@ -1,0 +1,143 @@
+"""ctypes mirror of include/voidray_hip.h and the loader of libvoidray_hip.so.
+
+The library is loaded from this package directory only (built in-tree by
+voidray_amd.build).  There is no fallback: if it is missing, every entry
+point raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoidray_hip.so")
+
+VR_OK = 0
+VR_MAT_LAMBERTIAN, VR_MAT_METAL, VR_MAT_DIELECTRIC, VR_MAT_EMISSION, VR_MAT_LAMBERTIAN_BSDF = range(5)
+VR_TEX_RGB32F, VR_TEX_RGBA8 = 0, 1
+VR_SAMPLE_NEAREST, VR_SAMPLE_BILINEAR = 0, 1
+VR_ENV_NONE, VR_ENV_UNIFORM, VR_ENV_HDRI = 0, 1, 2
+VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF = 0, 1
+VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
+VR_ENGINE_WIDE, VR_ENGINE_REFERENCE = 0, 1
+VR_MODE_FULL, VR_MODE_NORMAL = 0, 1
+VR_RENDER_ROWS_BY_HEIGHT = 1
+VR_RENDER_ENV_IS = 2
+VR_RENDER_RUSSIAN_ROULETTE = 4
+VR_RENDER_ALPHA_PER_CALL = 8
+VR_TONEMAP_NONE, VR_TONEMAP_ACES, VR_TONEMAP_REINHARD, VR_TONEMAP_FILMIC, VR_TONEMAP_UNCHARTED2 = range(5)
+
+F3 = C.c_float * 3
+
+
+class vr_mesh(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("positions", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p),
+                ("indices", C.c_void_p)]
+
+
+class vr_material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("color", F3), ("param", C.c_float),
+                ("albedo_texture", C.c_int32), ("normal_texture", C.c_int32)]
+
+
+class vr_texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_int32),
+                ("sample_type", C.c_int32), ("data", C.c_void_p)]
+
+
+class vr_environment(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("color", F3), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("data", C.c_void_p)]
+
+
+class vr_camera(C.Structure):
+    _fields_ = [("eye", F3), ("direction", F3), ("up", F3), ("fov", C.c_float), ("has_dof", C.c_int32),
+                ("aperture", C.c_float), ("focal_point", F3)]
+
+
+class vr_object(C.Structure):
+    _fields_ = [("surface", C.c_uint32), ("material", C.c_uint32)]
+
+
+class vr_scene_desc(C.Structure):
+    _fields_ = [("num_meshes", C.c_uint32), ("meshes", C.POINTER(vr_mesh)),
+                ("num_surfaces", C.c_uint32), ("surface_mesh", C.c_void_p),
+                ("num_objects", C.c_uint32), ("objects", C.POINTER(vr_object)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(vr_material)),
+                ("num_textures", C.c_uint32), ("textures", C.POINTER(vr_texture)),
+                ("environment", vr_environment), ("camera", vr_camera), ("build_flags", C.c_uint32)]
+
+
+class vr_scene_stats(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint32), ("num_surfaces", C.c_uint32), ("num_qnodes", C.c_uint32),
+                ("num_ref_nodes", C.c_uint32), ("qbvh_depth", C.c_uint32), ("max_leaf_size", C.c_uint32),
+                ("build_ref_seconds", C.c_double), ("build_sah_seconds", C.c_double),
+                ("upload_seconds", C.c_double), ("device_bytes", C.c_uint64)]
+
+
+class vr_trace_desc(C.Structure):
+    _fields_ = [("num_rays", C.c_uint32), ("t_min", C.c_float), ("mode", C.c_int32), ("engine", C.c_int32)]
+
+
+class vr_render_desc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("total_samples", C.c_uint32), ("sample_offset", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("firefly_clamp", C.c_float), ("render_mode", C.c_int32), ("flags", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("request", C.c_uint32), ("engine", C.c_uint32)]
+
+
+class vr_render_stats(C.Structure):
+    _fields_ = [("segments", C.c_uint64), ("samples", C.c_uint64), ("env_misses", C.c_uint64),
+                ("fallback_rays", C.c_uint64), ("capped_loops", C.c_uint64), ("kernel_ms", C.c_double)]
+
+
+class vr_post_desc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("scale", C.c_float), ("gamma", C.c_float),
+                ("exposure", C.c_float), ("tonemap", C.c_int32), ("clip_to_1024", C.c_uint32)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "vr_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "vr_destroy": (C.c_int, [_P]),
+    "vr_last_error": (C.c_char_p, []),
+    "vr_stream": (_P, [_P]),
+    "vr_scene_upload": (C.c_int, [_P, C.POINTER(vr_scene_desc), C.POINTER(_P)]),
+    "vr_scene_release": (C.c_int, [_P]),
+    "vr_scene_get_stats": (C.c_int, [_P, C.POINTER(vr_scene_stats)]),
+    "vr_trace": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P]),
+    "vr_trace_device_async": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_trace_last_fallbacks": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "vr_render": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats)]),
+    "vr_render_device_async": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats), _P]),
+    "vr_tonemap": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P]),
+    "vr_tonemap_device_async": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P, _P]),
+}
+
+_lib = None
+
+
+class VoidrayError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libvoidray_hip.so from the package directory (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VoidrayError(f"{LIB_PATH} is missing: build it with `python -m voidray_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != VR_OK:
+        msg = lib().vr_last_error()
+        raise VoidrayError(f"voidray error {rc}: {msg.decode() if msg else ''}")
+    return rc

@@ -1,0 +1,160 @@
+// Internal layout of a scene in HBM and the host-side build products.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/voidray_hip.h"
+#include "vrmath.h"
+
+namespace vr {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define VR_HIP_CHECK(expr)                                                                              \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            return ::vr::fail(VR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));           \
+    } while (0)
+
+// ---- wide quantised BVH node (64 B, one cache line pair of dwordx4 loads) --
+// Child boxes are stored as 8-bit offsets on a per-node, per-axis power-of-two
+// grid anchored at `origin`, rounded outward (lo floored, hi ceiled), so the
+// decoded box always contains the exact (padded) child box.
+struct alignas(16) QNode {
+    float ox, oy, oz;
+    uint8_t ex, ey, ez, nchild;  // biased exponents: scale = 2^(e - 127)
+    uint32_t child[4];           // internal: node index; leaf: kLeafBit | (count-1) << 27 | first
+    uint8_t qlo[3][4];           // [axis][child]
+    uint8_t qhi[3][4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
+constexpr uint32_t kMaxLeaf = 4;  // triangles per leaf (count-1 fits 3 bits)
+
+// Triangle record in leaf order (48 B): world-space vertices exactly as the
+// reference stores them, the global prim id and the reference tie rank.
+struct alignas(16) TriRec {
+    float p0[3], p1[3], p2[3];
+    uint32_t prim;
+    uint32_t rank;
+    uint32_t flags;  // bit 0: the owning mesh has a reference BVH (box tests apply); bits 1..31: surface
+};
+static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
+
+// Flattened reference median-split node (bvh.rs:40-45), pre-order: the left
+// child is the next node.  Leaf iff obj >= 0 (triangle index within the mesh,
+// or surface index in the top-level tree).
+struct alignas(16) RefNode {
+    float lo[3], hi[3];
+    int32_t right;
+    int32_t obj;
+};
+static_assert(sizeof(RefNode) == 32, "RefNode must be 32 bytes");
+
+struct MeshInfo {
+    int32_t root;          // index into ref_nodes, -1: small mesh (brute force)
+    uint32_t num_tris;
+    uint32_t tri_base;     // first triangle of the mesh in the concatenated mesh arrays
+    uint32_t vert_base;    // first vertex of the mesh in the concatenated vertex arrays
+};
+
+struct SurfaceInfo {
+    uint32_t mesh;
+    uint32_t prim_base;    // global prim id of the surface's first triangle
+    uint32_t rank_base;    // tie-rank offset (top-level in-order position)
+    uint32_t material;     // objects[surface].material (scene.rs:184)
+    uint32_t path_off;     // top-level ancestors: indices into top_path
+    uint32_t path_len;
+};
+
+struct GpuMaterial {
+    int32_t kind;
+    float color[3];
+    float param;
+    int32_t albedo_tex;
+    int32_t normal_tex;
+    int32_t pad;
+};
+
+struct GpuTexture {
+    uint32_t w, h;
+    int32_t format, sample_type;
+    uint64_t offset;  // byte offset into tex_data
+};
+
+// Device view of a scene, passed by value to kernels.
+struct SceneView {
+    const QNode* qnodes;
+    const TriRec* tris;
+    const RefNode* ref_nodes;     // all mesh trees
+    const RefNode* top_nodes;     // top-level tree over surfaces
+    const uint32_t* top_path;     // per-surface ancestor lists (top_nodes indices)
+    const MeshInfo* meshes;
+    const SurfaceInfo* surfaces;
+    const uint32_t* mesh_rank;    // per mesh triangle (concatenated): in-order rank within the mesh
+    const uint32_t* mesh_idx;     // per mesh triangle: 3 vertex indices (mesh-local)
+    const float* face_n;          // per mesh triangle: face normal (mesh.rs:80-84)
+    const float* vpos;            // concatenated vertices
+    const float* vnrm;
+    const float* vuv;
+    const uint32_t* prim_surface; // global prim -> surface
+    const GpuMaterial* materials;
+    const GpuTexture* textures;
+    const uint8_t* tex_data;
+    const float* env;             // HDRI RGB f32
+    const float* env_marg_cdf;    // env importance sampling tables (f32 copies of FP64 prefix sums)
+    const float* env_cond_cdf;
+    const float* env_pdf;         // per texel pdf (solid-angle measure factor folded in at use)
+    uint32_t num_qnodes, num_tris, num_surfaces, top_root_valid;
+    int32_t env_kind;
+    float env_color[3];
+    uint32_t env_w, env_h;
+    float env_total;              // sum of luminance * sin(theta) weights (FP64 computed)
+    // CameraAcceleration (camera.rs:57-67)
+    float cam_o[3], cam_dir[3], cam_right[3], cam_up[3];
+    float cam_d;
+    int32_t dof;
+    float aperture, focal_length;
+};
+
+}  // namespace vr
+
+struct vr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct vr_scene {
+    vr_ctx* ctx = nullptr;
+    vr::SceneView view{};
+    std::vector<void*> allocations;
+    vr_scene_stats stats{};
+    // scratch for traces (grown on demand)
+    uint32_t* fallback_list = nullptr;
+    uint32_t* counters = nullptr;  // [0] fallback count, [1..] misc
+    uint32_t* host_counters = nullptr;  // pinned mirror of counters, read after a sync
+    uint32_t fallback_cap = 0;
+    uint32_t last_fallbacks = 0;
+    // host copies used by the render driver
+    uint32_t num_prims = 0;
+    // render scratch (grown on demand)
+    void* render_scratch = nullptr;
+    size_t render_scratch_bytes = 0;
+};
+
+namespace vr {
+// scene.cpp
+int build_scene(vr_ctx* ctx, const vr_scene_desc* desc, vr_scene* s);
+void free_scene(vr_scene* s);
+// trace.hip
+int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d, uint32_t* d_prim,
+                 float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream);
+int ensure_trace_scratch(vr_scene* s, uint32_t n);
+}  // namespace vr

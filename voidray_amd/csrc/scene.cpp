@@ -1,0 +1,322 @@
+// Scene flattening + upload: the MI355X counterpart of
+// Scene::build_acceleration (voidray_renderer/src/core/scene.rs:163-179) and
+// the per-mesh BVH build of Mesh::from_buffers (core/mesh.rs:76-116).
+//
+// HBM layout (all arrays 256-byte aligned, one allocation each):
+//   qnodes   QNode[num_qnodes]      64 B  SAH 4-wide, 8-bit outward-quantised
+//   tris     TriRec[T]              48 B  world vertices in leaf order + prim + rank
+//   ref/top  RefNode[]              32 B  reference median-split trees (exact engine)
+//   per mesh triangle: idx (12 B), face normal (12 B), in-order rank (4 B)
+//   per vertex: position, normal (12 B each), uv (8 B)
+//   materials, textures (RGBA8 or RGB f32 texels), HDRI (RGB f32) + IS tables
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "bvh_build.h"
+#include "envmap.h"
+#include "internal.h"
+
+namespace vr {
+
+namespace {
+template <class T> int upload(vr_scene* s, const std::vector<T>& host, const T** dev) {
+    *dev = nullptr;
+    if (host.empty()) return 0;
+    void* p = nullptr;
+    size_t bytes = host.size() * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(VR_ERR_OOM, "hipMalloc failed for scene array");
+    s->allocations.push_back(p);
+    s->stats.device_bytes += bytes;
+    VR_HIP_CHECK(hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice));
+    *dev = (const T*)p;
+    return 0;
+}
+
+vf3 ld3(const float* p, size_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+}  // namespace
+
+int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
+    using clk = std::chrono::steady_clock;
+    s->ctx = ctx;
+    SceneView& v = s->view;
+    if (!d) return fail(VR_ERR_INVALID, "null scene description");
+    if (d->num_surfaces > 0 && (!d->surface_mesh || !d->meshes)) return fail(VR_ERR_INVALID, "surfaces without meshes");
+    if (d->num_objects < d->num_surfaces)
+        return fail(VR_ERR_INVALID, "objects[] must cover every surface index (scene.rs:184 looks up objects[surface])");
+    for (uint32_t i = 0; i < d->num_surfaces; ++i)
+        if (d->surface_mesh[i] >= d->num_meshes) return fail(VR_ERR_INVALID, "surface_mesh index out of range");
+    for (uint32_t i = 0; i < d->num_surfaces; ++i)
+        if (d->objects[i].material >= d->num_materials) return fail(VR_ERR_INVALID, "object material out of range");
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const vr_material& m = d->materials[i];
+        if (m.kind < 0 || m.kind > VR_MAT_LAMBERTIAN_BSDF) return fail(VR_ERR_INVALID, "unknown material kind");
+        if (m.albedo_texture >= (int32_t)d->num_textures || m.normal_texture >= (int32_t)d->num_textures)
+            return fail(VR_ERR_INVALID, "material texture index out of range");
+    }
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        const vr_texture& t = d->textures[i];
+        if (!t.data || t.width == 0 || t.height == 0) return fail(VR_ERR_INVALID, "empty texture");
+        if (t.format != VR_TEX_RGB32F && t.format != VR_TEX_RGBA8) return fail(VR_ERR_INVALID, "unknown texture format");
+    }
+    auto t_ref0 = clk::now();
+
+    // ---- meshes: concatenated vertices / triangles, reference trees -------
+    uint32_t M = d->num_meshes;
+    std::vector<MeshInfo> minfo(M);
+    std::vector<float> vpos, vnrm, vuv, face_n;
+    std::vector<uint32_t> midx, mrank;
+    std::vector<RefNode> ref_nodes;
+    std::vector<Box3> mesh_bounds(M);
+    uint64_t tv = 0, tt = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        tv += d->meshes[m].num_vertices;
+        tt += d->meshes[m].num_triangles;
+    }
+    if (tv >= (1ull << 32) || tt >= (1ull << 31)) return fail(VR_ERR_INVALID, "scene too large");
+    vpos.reserve(3 * tv);
+    vnrm.reserve(3 * tv);
+    vuv.reserve(2 * tv);
+    midx.reserve(3 * tt);
+    face_n.reserve(3 * tt);
+    mrank.reserve(tt);
+    for (uint32_t m = 0; m < M; ++m) {
+        const vr_mesh& src = d->meshes[m];
+        if (src.num_vertices && !src.positions) return fail(VR_ERR_INVALID, "mesh without positions");
+        if (src.num_triangles && !src.indices) return fail(VR_ERR_INVALID, "mesh without indices");
+        MeshInfo& mi = minfo[m];
+        mi.vert_base = (uint32_t)(vpos.size() / 3);
+        mi.tri_base = (uint32_t)(midx.size() / 3);
+        mi.num_tris = src.num_triangles;
+        mi.root = -1;
+        Box3 mb{mk3(kInf, kInf, kInf), mk3(-kInf, -kInf, -kInf)};  // AABB::default
+        for (uint32_t i = 0; i < src.num_vertices; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                vpos.push_back(src.positions[3 * i + a]);
+                vnrm.push_back(src.normals ? src.normals[3 * i + a] : 0.0f);
+            }
+            vuv.push_back(src.uvs ? src.uvs[2 * i] : 0.0f);
+            vuv.push_back(src.uvs ? src.uvs[2 * i + 1] : 0.0f);
+            vf3 p = ld3(src.positions, i);
+            mb.lo = mk3(fminf(mb.lo.x, p.x), fminf(mb.lo.y, p.y), fminf(mb.lo.z, p.z));
+            mb.hi = mk3(fmaxf(mb.hi.x, p.x), fmaxf(mb.hi.y, p.y), fmaxf(mb.hi.z, p.z));
+        }
+        mesh_bounds[m] = mb;
+        std::vector<Box3> tb(src.num_triangles);
+        for (uint32_t t = 0; t < src.num_triangles; ++t) {
+            uint32_t i0 = src.indices[3 * t], i1 = src.indices[3 * t + 1], i2 = src.indices[3 * t + 2];
+            if (i0 >= src.num_vertices || i1 >= src.num_vertices || i2 >= src.num_vertices)
+                return fail(VR_ERR_INVALID, "triangle index out of range");
+            midx.push_back(i0);
+            midx.push_back(i1);
+            midx.push_back(i2);
+            vf3 p0 = ld3(src.positions, i0), p1 = ld3(src.positions, i1), p2 = ld3(src.positions, i2);
+            // face normal (mesh.rs:80-84)
+            vf3 fn = normalize3(cross3(p2 - p1, p0 - p1));
+            face_n.push_back(fn.x);
+            face_n.push_back(fn.y);
+            face_n.push_back(fn.z);
+            tb[t] = tri_box_eps(p0, p1, p2);
+        }
+        if (src.num_triangles > 4) {  // SMALL_MESH (mesh.rs:43, 111-114)
+            std::vector<RefNode> tree;
+            std::vector<uint32_t> rank;
+            build_reference_tree(tb, tree, rank);
+            int32_t off = (int32_t)ref_nodes.size();
+            for (auto& nd : tree)
+                if (nd.obj < 0) nd.right += off;
+            mi.root = off;
+            ref_nodes.insert(ref_nodes.end(), tree.begin(), tree.end());
+            mrank.insert(mrank.end(), rank.begin(), rank.end());
+        } else {
+            // brute-force loop keeps the earlier triangle on equal t (mesh.rs:131)
+            for (uint32_t t = 0; t < src.num_triangles; ++t) mrank.push_back(src.num_triangles - 1 - t);
+        }
+    }
+
+    // ---- top-level tree over surfaces (scene.rs:72-92, 176) ---------------
+    uint32_t S = d->num_surfaces;
+    std::vector<Box3> sb(S);
+    for (uint32_t i = 0; i < S; ++i) sb[i] = mesh_bounds[d->surface_mesh[i]];
+    std::vector<RefNode> top;
+    std::vector<uint32_t> top_rank;
+    build_reference_tree(sb, top, top_rank);
+    std::vector<SurfaceInfo> sinfo(S);
+    std::vector<uint32_t> order(S);
+    for (uint32_t i = 0; i < S; ++i) order[top_rank[i]] = i;
+    uint32_t rb = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+        uint32_t sidx = order[k];
+        sinfo[sidx].rank_base = rb;
+        rb += minfo[d->surface_mesh[sidx]].num_tris;
+    }
+    uint32_t pb = 0;
+    for (uint32_t i = 0; i < S; ++i) {
+        sinfo[i].mesh = d->surface_mesh[i];
+        sinfo[i].prim_base = pb;
+        sinfo[i].material = d->objects[i].material;
+        pb += minfo[d->surface_mesh[i]].num_tris;
+    }
+    uint32_t T = pb;
+    // ancestor lists of every surface leaf (root -> parent)
+    std::vector<uint32_t> top_path;
+    {
+        std::vector<std::pair<int32_t, std::vector<uint32_t>>> st;
+        if (!top.empty()) st.push_back({0, {}});
+        std::vector<std::vector<uint32_t>> paths(S);
+        while (!st.empty()) {
+            auto [ni, path] = st.back();
+            st.pop_back();
+            const RefNode& nd = top[ni];
+            if (nd.obj >= 0) {
+                paths[nd.obj] = path;
+                continue;
+            }
+            auto p2 = path;
+            p2.push_back((uint32_t)ni);
+            st.push_back({nd.right, p2});
+            st.push_back({ni + 1, p2});
+        }
+        for (uint32_t i = 0; i < S; ++i) {
+            sinfo[i].path_off = (uint32_t)top_path.size();
+            sinfo[i].path_len = (uint32_t)paths[i].size();
+            top_path.insert(top_path.end(), paths[i].begin(), paths[i].end());
+        }
+    }
+    auto t_ref1 = clk::now();
+
+    // ---- fast structure: triangle records + SAH quantised BVH4 ------------
+    std::vector<TriRec> recs(T);
+    std::vector<uint32_t> prim_surface(T);
+    for (uint32_t i = 0; i < S; ++i) {
+        const MeshInfo& mi = minfo[sinfo[i].mesh];
+        for (uint32_t t = 0; t < mi.num_tris; ++t) {
+            TriRec& r = recs[sinfo[i].prim_base + t];
+            const uint32_t* ix = &midx[3 * (size_t)(mi.tri_base + t)];
+            for (int a = 0; a < 3; ++a) {
+                r.p0[a] = vpos[3 * (size_t)(mi.vert_base + ix[0]) + a];
+                r.p1[a] = vpos[3 * (size_t)(mi.vert_base + ix[1]) + a];
+                r.p2[a] = vpos[3 * (size_t)(mi.vert_base + ix[2]) + a];
+            }
+            r.prim = sinfo[i].prim_base + t;
+            r.rank = sinfo[i].rank_base + mrank[mi.tri_base + t];
+            r.flags = (mi.root >= 0 ? 1u : 0u) | (i << 1);
+            prim_surface[sinfo[i].prim_base + t] = i;
+        }
+    }
+    std::vector<QNode> qn;
+    std::vector<TriRec> tris;
+    uint32_t depth = 0;
+    std::string err;
+    if (build_qbvh(recs, qn, tris, depth, err) != 0) return fail(VR_ERR_INVALID, err);
+    std::vector<TriRec>().swap(recs);
+    auto t_sah1 = clk::now();
+
+    // ---- materials / textures / environment / camera ----------------------
+    std::vector<GpuMaterial> mats(d->num_materials);
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const vr_material& m = d->materials[i];
+        mats[i] = GpuMaterial{m.kind, {m.color[0], m.color[1], m.color[2]}, m.param, m.albedo_texture,
+                              m.normal_texture, 0};
+    }
+    std::vector<GpuTexture> texs(d->num_textures);
+    std::vector<uint8_t> texdata;
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        const vr_texture& t = d->textures[i];
+        size_t texel = t.format == VR_TEX_RGBA8 ? 4 : 12;
+        size_t bytes = texel * t.width * (size_t)t.height;
+        size_t off = (texdata.size() + 255) & ~(size_t)255;
+        texdata.resize(off + bytes);
+        std::memcpy(texdata.data() + off, t.data, bytes);
+        texs[i] = GpuTexture{t.width, t.height, t.format, t.sample_type, off};
+    }
+    v.env_kind = d->environment.kind;
+    for (int a = 0; a < 3; ++a) v.env_color[a] = d->environment.color[a];
+    std::vector<float> env, env_marg, env_cond, env_pdf;
+    if (v.env_kind == VR_ENV_HDRI) {
+        const vr_environment& e = d->environment;
+        if (!e.data || e.width == 0 || e.height == 0) return fail(VR_ERR_INVALID, "empty HDRI environment");
+        env.assign(e.data, e.data + 3 * (size_t)e.width * e.height);
+        v.env_w = e.width;
+        v.env_h = e.height;
+        if (!(d->build_flags & VR_BUILD_NO_ENV_CDF)) {
+            double total = build_env_tables(env.data(), e.width, e.height, env_marg, env_cond, env_pdf);
+            v.env_total = (float)total;
+        }
+    } else if (v.env_kind != VR_ENV_NONE && v.env_kind != VR_ENV_UNIFORM) {
+        return fail(VR_ERR_INVALID, "unknown environment kind");
+    }
+    const vr_camera& c = d->camera;
+    vf3 dir = mk3(c.direction[0], c.direction[1], c.direction[2]);
+    vf3 up = mk3(c.up[0], c.up[1], c.up[2]);
+    vf3 eye = mk3(c.eye[0], c.eye[1], c.eye[2]);
+    vf3 right = normalize3(cross3(dir, up));
+    for (int a = 0; a < 3; ++a) {
+        v.cam_o[a] = c.eye[a];
+        v.cam_dir[a] = c.direction[a];
+        v.cam_up[a] = c.up[a];
+        v.cam_right[a] = comp(right, a);
+    }
+    v.cam_d = 1.0f / std::tan(c.fov / 2.0f);  // (fov / 2).tan().recip()  (camera.rs:39)
+    v.dof = c.has_dof;
+    v.aperture = c.aperture;
+    v.focal_length = dot3(mk3(c.focal_point[0], c.focal_point[1], c.focal_point[2]) - eye, dir);
+
+    // ---- upload --------------------------------------------------------------
+    auto t_up0 = clk::now();
+    int rc = 0;
+    rc |= upload(s, qn, &v.qnodes);
+    rc |= upload(s, tris, &v.tris);
+    rc |= upload(s, ref_nodes, &v.ref_nodes);
+    rc |= upload(s, top, &v.top_nodes);
+    rc |= upload(s, top_path, &v.top_path);
+    rc |= upload(s, minfo, &v.meshes);
+    rc |= upload(s, sinfo, &v.surfaces);
+    rc |= upload(s, mrank, &v.mesh_rank);
+    rc |= upload(s, midx, &v.mesh_idx);
+    rc |= upload(s, face_n, &v.face_n);
+    rc |= upload(s, vpos, &v.vpos);
+    rc |= upload(s, vnrm, &v.vnrm);
+    rc |= upload(s, vuv, &v.vuv);
+    rc |= upload(s, prim_surface, &v.prim_surface);
+    rc |= upload(s, mats, &v.materials);
+    rc |= upload(s, texs, &v.textures);
+    rc |= upload(s, texdata, &v.tex_data);
+    rc |= upload(s, env, &v.env);
+    rc |= upload(s, env_marg, &v.env_marg_cdf);
+    rc |= upload(s, env_cond, &v.env_cond_cdf);
+    rc |= upload(s, env_pdf, &v.env_pdf);
+    if (rc) return VR_ERR_OOM;
+    v.num_qnodes = (uint32_t)qn.size();
+    v.num_tris = T;
+    v.num_surfaces = S;
+    v.top_root_valid = top.empty() ? 0 : 1;
+    s->num_prims = T;
+    auto t_up1 = clk::now();
+
+    vr_scene_stats& st = s->stats;
+    st.num_triangles = T;
+    st.num_surfaces = S;
+    st.num_qnodes = (uint32_t)qn.size();
+    st.num_ref_nodes = (uint32_t)ref_nodes.size();
+    st.qbvh_depth = depth;
+    st.max_leaf_size = kMaxLeaf;
+    st.build_ref_seconds = std::chrono::duration<double>(t_ref1 - t_ref0).count();
+    st.build_sah_seconds = std::chrono::duration<double>(t_sah1 - t_ref1).count();
+    st.upload_seconds = std::chrono::duration<double>(t_up1 - t_up0).count();
+    return 0;
+}
+
+void free_scene(vr_scene* s) {
+    for (void* p : s->allocations) (void)hipFree(p);
+    s->allocations.clear();
+    if (s->fallback_list) (void)hipFree(s->fallback_list);
+    if (s->counters) (void)hipFree(s->counters);
+    if (s->render_scratch) (void)hipFree(s->render_scratch);
+    s->fallback_list = nullptr;
+    s->counters = nullptr;
+    s->render_scratch = nullptr;
+}
+
+}  // namespace vr

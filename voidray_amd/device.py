@@ -1,0 +1,134 @@
+"""Device-side objects: context, uploaded scene, ray queries, rendering."""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class Context:
+    _default = None
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        N.check(N.lib().vr_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    @property
+    def stream(self):
+        return N.lib().vr_stream(self.handle)
+
+    def close(self):
+        if self.handle:
+            N.lib().vr_destroy(self.handle)
+            self.handle = None
+
+
+class SceneAcceleration:
+    """An uploaded scene (Scene::build_acceleration, core/scene.rs:163-179)."""
+
+    def __init__(self, ctx, scene, build_flags=0):
+        self.ctx = ctx
+        self.scene = scene
+        desc, keep = scene.to_desc(build_flags)
+        h = C.c_void_p()
+        N.check(N.lib().vr_scene_upload(ctx.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.num_prims = scene.num_triangles
+
+    def stats(self):
+        s = N.vr_scene_stats()
+        N.check(N.lib().vr_scene_get_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in N.vr_scene_stats._fields_}
+
+    def release(self):
+        if self.handle:
+            N.lib().vr_scene_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    # -- ray queries: SceneAcceleration::hit (core/scene.rs:182-185) ---------
+    def hit(self, origins, directions, t_min=1e-5, any_hit=False, engine=N.VR_ENGINE_WIDE):
+        """Host arrays in, host arrays out.  Returns dict prim/t/u/v (closest) or hit (any)."""
+        o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        n = len(o)
+        desc = N.vr_trace_desc(n, t_min, N.VR_TRACE_ANY if any_hit else N.VR_TRACE_CLOSEST, engine)
+        out = {}
+        if any_hit:
+            hit = np.zeros(n, np.uint8)
+            N.check(N.lib().vr_trace(self.handle, C.byref(desc), o.ctypes.data, d.ctypes.data, None, None, None,
+                                     None, hit.ctypes.data))
+            out["hit"] = hit
+        else:
+            prim = np.zeros(n, np.uint32)
+            t = np.zeros(n, np.float32)
+            u = np.zeros(n, np.float32)
+            v = np.zeros(n, np.float32)
+            N.check(N.lib().vr_trace(self.handle, C.byref(desc), o.ctypes.data, d.ctypes.data, prim.ctypes.data,
+                                     t.ctypes.data, u.ctypes.data, v.ctypes.data, None))
+            out.update(prim=prim, t=t, u=u, v=v)
+        return out
+
+    def last_fallbacks(self):
+        c = C.c_uint32()
+        N.check(N.lib().vr_trace_last_fallbacks(self.handle, C.byref(c)))
+        return c.value
+
+    def trace_device(self, d_origins, d_directions, n, any_hit=False, t_min=1e-5, engine=N.VR_ENGINE_WIDE,
+                     d_prim=None, d_t=None, d_u=None, d_v=None, d_hit=None, stream=None):
+        """Device pointers (ints) in/out, enqueued on `stream` (None = context stream)."""
+        desc = N.vr_trace_desc(n, t_min, N.VR_TRACE_ANY if any_hit else N.VR_TRACE_CLOSEST, engine)
+        N.check(N.lib().vr_trace_device_async(self.handle, C.byref(desc), d_origins, d_directions, d_prim, d_t,
+                                              d_u, d_v, d_hit, stream))
+
+    # -- rendering ---------------------------------------------------------------
+    def render(self, rgba, desc):
+        """rgba: (H, W, 4) float32 host array, accumulated in place."""
+        assert rgba.dtype == np.float32 and rgba.flags.c_contiguous
+        st = N.vr_render_stats()
+        N.check(N.lib().vr_render(self.handle, C.byref(desc), rgba.ctypes.data, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in N.vr_render_stats._fields_}
+
+    def render_device(self, d_rgba, desc, stream=None):
+        st = N.vr_render_stats()
+        N.check(N.lib().vr_render_device_async(self.handle, C.byref(desc), d_rgba, C.byref(st), stream))
+        return st
+
+
+def make_render_desc(width, height, samples, total_samples, settings=None, sample_offset=0, seed=0, request=0,
+                     flags=N.VR_RENDER_ALPHA_PER_CALL, rr_depth=0, engine=N.VR_ENGINE_WIDE):
+    from .scene import RenderSettings
+    s = settings or RenderSettings()
+    return N.vr_render_desc(width, height, samples, total_samples, sample_offset, s.max_bounces,
+                            float(s.firefly_clamp), s.render_mode, flags, rr_depth, seed, request, engine)
+
+
+def iterative_render(target, scene_accel, settings, samples, sample_offset=0, seed=0, request=0,
+                     flags=N.VR_RENDER_ROWS_BY_HEIGHT | N.VR_RENDER_ALPHA_PER_CALL):
+    """render/iterative.rs:11-55: adds (sum of `samples` traces) / total_samples into target
+    ((H, W, 4) float32).  `sample_offset` selects the sample indices of the per-pixel stream."""
+    h, w = target.shape[:2]
+    desc = make_render_desc(w, h, samples, settings.total_samples, settings, sample_offset, seed, request, flags)
+    return scene_accel.render(target, desc)
+
+
+def tonemap(ctx, src, post, dst=None):
+    """post_process.glsl over an (H, W, 4) float32 image."""
+    h, w = src.shape[:2]
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    dst = np.zeros_like(src) if dst is None else dst
+    N.check(N.lib().vr_tonemap(ctx.handle, C.byref(post), src.ctypes.data, dst.ctypes.data))
+    return dst
