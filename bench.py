@@ -92,7 +92,9 @@ def main():
     tu = torch.empty(n, dtype=torch.float32, device=dev)
     tv = torch.empty(n, dtype=torch.float32, device=dev)
     thit = torch.empty(n, dtype=torch.uint8, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    # a dedicated stream (default priority): kernels and timing events share it
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
 
     def step(lo=0, cnt=n, any_hit=False):

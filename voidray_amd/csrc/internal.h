@@ -150,7 +150,25 @@ struct vr_scene {
 };
 
 namespace vr {
+// Host-side build products of a scene (the arrays that go to HBM).  `view`
+// points into these vectors, so the traversal routines can also run on the
+// host over the same data.
+struct HostScene {
+    std::vector<QNode> qnodes;
+    std::vector<TriRec> tris;
+    std::vector<RefNode> ref_nodes, top_nodes;
+    std::vector<uint32_t> top_path, mesh_rank, mesh_idx, prim_surface;
+    std::vector<MeshInfo> meshes;
+    std::vector<SurfaceInfo> surfaces;
+    std::vector<float> face_n, vpos, vnrm, vuv, env, env_marg, env_cond, env_pdf;
+    std::vector<GpuMaterial> materials;
+    std::vector<GpuTexture> textures;
+    std::vector<uint8_t> tex_data;
+    SceneView view{};
+    vr_scene_stats stats{};
+};
 // scene.cpp
+int build_host_scene(const vr_scene_desc* desc, HostScene& h);
 int build_scene(vr_ctx* ctx, const vr_scene_desc* desc, vr_scene* s);
 void free_scene(vr_scene* s);
 // trace.hip

@@ -43,13 +43,13 @@ __device__ __forceinline__ float tm_u2(float c) {
     return x / white;
 }
 
-__global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const float4* __restrict__ src, float4* __restrict__ dst) {
+__global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const f32x4* __restrict__ src, f32x4* __restrict__ dst) {
     uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
     uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= p.width || y >= p.height) return;
     if (p.clip_to_1024 && (x >= 1024 || y >= 1024)) return;
     size_t i = (size_t)y * p.width + x;
-    float4 s = src[i];
+    f32x4 s = src[i];
     float ex = powf(2.f, p.exposure);
     float r = (s.x * p.scale) * ex, g = (s.y * p.scale) * ex, b = (s.z * p.scale) * ex;
     switch (p.tonemap) {
@@ -68,14 +68,14 @@ __global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const float4* _
         default: break;
     }
     float ig = 1.f / p.gamma;
-    dst[i] = make_float4(powf(r, ig), powf(g, ig), powf(b, ig), 1.f);
+    dst[i] = f32x4{powf(r, ig), powf(g, ig), powf(b, ig), 1.f};
 }
 
 int tonemap_device(const vr_post_desc* desc, const float* d_src, float* d_dst, hipStream_t st) {
     if (desc->width == 0 || desc->height == 0) return 0;
     if (((uintptr_t)d_src & 15) || ((uintptr_t)d_dst & 15)) return fail(VR_ERR_INVALID, "buffers must be 16-byte aligned");
     dim3 grid((desc->width + 15) / 16, (desc->height + 15) / 16);
-    hipLaunchKernelGGL(k_tonemap, grid, dim3(256), 0, st, *desc, (const float4*)d_src, (float4*)d_dst);
+    hipLaunchKernelGGL(k_tonemap, grid, dim3(256), 0, st, *desc, (const f32x4*)d_src, (f32x4*)d_dst);
     VR_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -37,10 +37,9 @@ template <class T> int upload(vr_scene* s, const std::vector<T>& host, const T**
 vf3 ld3(const float* p, size_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 }  // namespace
 
-int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
+int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     using clk = std::chrono::steady_clock;
-    s->ctx = ctx;
-    SceneView& v = s->view;
+    SceneView& v = H.view;
     if (!d) return fail(VR_ERR_INVALID, "null scene description");
     if (d->num_surfaces > 0 && (!d->surface_mesh || !d->meshes)) return fail(VR_ERR_INVALID, "surfaces without meshes");
     if (d->num_objects < d->num_surfaces)
@@ -64,10 +63,11 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
 
     // ---- meshes: concatenated vertices / triangles, reference trees -------
     uint32_t M = d->num_meshes;
-    std::vector<MeshInfo> minfo(M);
-    std::vector<float> vpos, vnrm, vuv, face_n;
-    std::vector<uint32_t> midx, mrank;
-    std::vector<RefNode> ref_nodes;
+    std::vector<MeshInfo>& minfo = H.meshes;
+    minfo.assign(M, MeshInfo{});
+    std::vector<float>&vpos = H.vpos, &vnrm = H.vnrm, &vuv = H.vuv, &face_n = H.face_n;
+    std::vector<uint32_t>&midx = H.mesh_idx, &mrank = H.mesh_rank;
+    std::vector<RefNode>& ref_nodes = H.ref_nodes;
     std::vector<Box3> mesh_bounds(M);
     uint64_t tv = 0, tt = 0;
     for (uint32_t m = 0; m < M; ++m) {
@@ -139,10 +139,11 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     uint32_t S = d->num_surfaces;
     std::vector<Box3> sb(S);
     for (uint32_t i = 0; i < S; ++i) sb[i] = mesh_bounds[d->surface_mesh[i]];
-    std::vector<RefNode> top;
+    std::vector<RefNode>& top = H.top_nodes;
     std::vector<uint32_t> top_rank;
     build_reference_tree(sb, top, top_rank);
-    std::vector<SurfaceInfo> sinfo(S);
+    std::vector<SurfaceInfo>& sinfo = H.surfaces;
+    sinfo.assign(S, SurfaceInfo{});
     std::vector<uint32_t> order(S);
     for (uint32_t i = 0; i < S; ++i) order[top_rank[i]] = i;
     uint32_t rb = 0;
@@ -160,7 +161,7 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     }
     uint32_t T = pb;
     // ancestor lists of every surface leaf (root -> parent)
-    std::vector<uint32_t> top_path;
+    std::vector<uint32_t>& top_path = H.top_path;
     {
         std::vector<std::pair<int32_t, std::vector<uint32_t>>> st;
         if (!top.empty()) st.push_back({0, {}});
@@ -188,7 +189,8 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
 
     // ---- fast structure: triangle records + SAH quantised BVH4 ------------
     std::vector<TriRec> recs(T);
-    std::vector<uint32_t> prim_surface(T);
+    std::vector<uint32_t>& prim_surface = H.prim_surface;
+    prim_surface.assign(T, 0);
     for (uint32_t i = 0; i < S; ++i) {
         const MeshInfo& mi = minfo[sinfo[i].mesh];
         for (uint32_t t = 0; t < mi.num_tris; ++t) {
@@ -205,8 +207,8 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
             prim_surface[sinfo[i].prim_base + t] = i;
         }
     }
-    std::vector<QNode> qn;
-    std::vector<TriRec> tris;
+    std::vector<QNode>& qn = H.qnodes;
+    std::vector<TriRec>& tris = H.tris;
     uint32_t depth = 0;
     std::string err;
     if (build_qbvh(recs, qn, tris, depth, err) != 0) return fail(VR_ERR_INVALID, err);
@@ -214,14 +216,16 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     auto t_sah1 = clk::now();
 
     // ---- materials / textures / environment / camera ----------------------
-    std::vector<GpuMaterial> mats(d->num_materials);
+    std::vector<GpuMaterial>& mats = H.materials;
+    mats.assign(d->num_materials, GpuMaterial{});
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const vr_material& m = d->materials[i];
         mats[i] = GpuMaterial{m.kind, {m.color[0], m.color[1], m.color[2]}, m.param, m.albedo_texture,
                               m.normal_texture, 0};
     }
-    std::vector<GpuTexture> texs(d->num_textures);
-    std::vector<uint8_t> texdata;
+    std::vector<GpuTexture>& texs = H.textures;
+    texs.assign(d->num_textures, GpuTexture{});
+    std::vector<uint8_t>& texdata = H.tex_data;
     for (uint32_t i = 0; i < d->num_textures; ++i) {
         const vr_texture& t = d->textures[i];
         size_t texel = t.format == VR_TEX_RGBA8 ? 4 : 12;
@@ -233,7 +237,7 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     }
     v.env_kind = d->environment.kind;
     for (int a = 0; a < 3; ++a) v.env_color[a] = d->environment.color[a];
-    std::vector<float> env, env_marg, env_cond, env_pdf;
+    std::vector<float>&env = H.env, &env_marg = H.env_marg, &env_cond = H.env_cond, &env_pdf = H.env_pdf;
     if (v.env_kind == VR_ENV_HDRI) {
         const vr_environment& e = d->environment;
         if (!e.data || e.width == 0 || e.height == 0) return fail(VR_ERR_INVALID, "empty HDRI environment");
@@ -263,39 +267,34 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     v.aperture = c.aperture;
     v.focal_length = dot3(mk3(c.focal_point[0], c.focal_point[1], c.focal_point[2]) - eye, dir);
 
-    // ---- upload --------------------------------------------------------------
-    auto t_up0 = clk::now();
-    int rc = 0;
-    rc |= upload(s, qn, &v.qnodes);
-    rc |= upload(s, tris, &v.tris);
-    rc |= upload(s, ref_nodes, &v.ref_nodes);
-    rc |= upload(s, top, &v.top_nodes);
-    rc |= upload(s, top_path, &v.top_path);
-    rc |= upload(s, minfo, &v.meshes);
-    rc |= upload(s, sinfo, &v.surfaces);
-    rc |= upload(s, mrank, &v.mesh_rank);
-    rc |= upload(s, midx, &v.mesh_idx);
-    rc |= upload(s, face_n, &v.face_n);
-    rc |= upload(s, vpos, &v.vpos);
-    rc |= upload(s, vnrm, &v.vnrm);
-    rc |= upload(s, vuv, &v.vuv);
-    rc |= upload(s, prim_surface, &v.prim_surface);
-    rc |= upload(s, mats, &v.materials);
-    rc |= upload(s, texs, &v.textures);
-    rc |= upload(s, texdata, &v.tex_data);
-    rc |= upload(s, env, &v.env);
-    rc |= upload(s, env_marg, &v.env_marg_cdf);
-    rc |= upload(s, env_cond, &v.env_cond_cdf);
-    rc |= upload(s, env_pdf, &v.env_pdf);
-    if (rc) return VR_ERR_OOM;
     v.num_qnodes = (uint32_t)qn.size();
     v.num_tris = T;
     v.num_surfaces = S;
     v.top_root_valid = top.empty() ? 0 : 1;
-    s->num_prims = T;
-    auto t_up1 = clk::now();
+    // host view (the traversal code can run over it on the CPU)
+    v.qnodes = qn.data();
+    v.tris = tris.data();
+    v.ref_nodes = ref_nodes.data();
+    v.top_nodes = top.data();
+    v.top_path = top_path.data();
+    v.meshes = minfo.data();
+    v.surfaces = sinfo.data();
+    v.mesh_rank = mrank.data();
+    v.mesh_idx = midx.data();
+    v.face_n = face_n.data();
+    v.vpos = vpos.data();
+    v.vnrm = vnrm.data();
+    v.vuv = vuv.data();
+    v.prim_surface = prim_surface.data();
+    v.materials = mats.data();
+    v.textures = texs.data();
+    v.tex_data = texdata.data();
+    v.env = env.data();
+    v.env_marg_cdf = env_marg.data();
+    v.env_cond_cdf = env_cond.data();
+    v.env_pdf = env_pdf.data();
 
-    vr_scene_stats& st = s->stats;
+    vr_scene_stats& st = H.stats;
     st.num_triangles = T;
     st.num_surfaces = S;
     st.num_qnodes = (uint32_t)qn.size();
@@ -304,7 +303,44 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     st.max_leaf_size = kMaxLeaf;
     st.build_ref_seconds = std::chrono::duration<double>(t_ref1 - t_ref0).count();
     st.build_sah_seconds = std::chrono::duration<double>(t_sah1 - t_ref1).count();
-    st.upload_seconds = std::chrono::duration<double>(t_up1 - t_up0).count();
+    return 0;
+}
+
+int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
+    s->ctx = ctx;
+    HostScene H;
+    int rc = build_host_scene(d, H);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    SceneView& v = s->view;
+    v = H.view;
+    s->stats = H.stats;
+    s->stats.device_bytes = 0;
+    rc = 0;
+    rc |= upload(s, H.qnodes, &v.qnodes);
+    rc |= upload(s, H.tris, &v.tris);
+    rc |= upload(s, H.ref_nodes, &v.ref_nodes);
+    rc |= upload(s, H.top_nodes, &v.top_nodes);
+    rc |= upload(s, H.top_path, &v.top_path);
+    rc |= upload(s, H.meshes, &v.meshes);
+    rc |= upload(s, H.surfaces, &v.surfaces);
+    rc |= upload(s, H.mesh_rank, &v.mesh_rank);
+    rc |= upload(s, H.mesh_idx, &v.mesh_idx);
+    rc |= upload(s, H.face_n, &v.face_n);
+    rc |= upload(s, H.vpos, &v.vpos);
+    rc |= upload(s, H.vnrm, &v.vnrm);
+    rc |= upload(s, H.vuv, &v.vuv);
+    rc |= upload(s, H.prim_surface, &v.prim_surface);
+    rc |= upload(s, H.materials, &v.materials);
+    rc |= upload(s, H.textures, &v.textures);
+    rc |= upload(s, H.tex_data, &v.tex_data);
+    rc |= upload(s, H.env, &v.env);
+    rc |= upload(s, H.env_marg, &v.env_marg_cdf);
+    rc |= upload(s, H.env_cond, &v.env_cond_cdf);
+    rc |= upload(s, H.env_pdf, &v.env_pdf);
+    if (rc) return VR_ERR_OOM;
+    s->num_prims = v.num_tris;
+    s->stats.upload_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }
 

@@ -6,7 +6,7 @@
 namespace vr {
 
 constexpr int kTraceBlock = 128;  // two wave64s
-constexpr int kTraceStack = 16;   // LDS stack entries per lane (8 B each)
+constexpr int kTraceStack = 32;   // LDS stack entries per lane (8 B each)
 
 struct TraceArgs {
     SceneView sv;
@@ -32,15 +32,16 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
     if (a.hit) a.hit[i] = h.prim != 0xFFFFFFFFu ? 1 : 0;
 }
 
+template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
-    __shared__ uint2 stk[kTraceStack * kTraceBlock];
+    __shared__ u32x2 stk[kTraceStack * kTraceBlock];
     uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= a.n) return;
     vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
     vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
     HitOut h;
-    int rc = trace_wide<kTraceStack>(a.sv, o, d, a.tmin, kInf, a.any != 0, stk + threadIdx.x, kTraceBlock, h);
-    if (rc == kWideFallback) {
+    int rc = trace_wide<kTraceStack, ANY>(a.sv, o, d, a.tmin, kInf, stk + threadIdx.x, kTraceBlock, h);
+    if (rc != kWideDone) {
         uint32_t slot = atomicAdd(a.fb_count, 1u);
         a.fb_list[slot] = i;
         return;
@@ -97,7 +98,10 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, s->counters, 64u);
     if (desc->engine == VR_ENGINE_WIDE) {
         dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
-        hipLaunchKernelGGL(k_trace_wide, grid, dim3(kTraceBlock), 0, stream, a);
+        if (a.any)
+            hipLaunchKernelGGL(k_trace_wide<true>, grid, dim3(kTraceBlock), 0, stream, a);
+        else
+            hipLaunchKernelGGL(k_trace_wide<false>, grid, dim3(kTraceBlock), 0, stream, a);
         // fallbacks: persistent grid reads the device-side count (no host sync)
         hipLaunchKernelGGL(k_trace_exact, dim3(1024), dim3(64), 0, stream, a, (const uint32_t*)s->fallback_list,
                            (const uint32_t*)s->counters);

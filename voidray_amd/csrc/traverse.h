@@ -36,20 +36,21 @@ constexpr float kCullSlack = 1.0f + 1.0f / 65536.0f;   // 2^-16 relative on t_be
 constexpr float kRobustRel = 1.0f / 1048576.0f;        // 2^-20 interior margin
 
 
-__device__ __forceinline__ void load_tri(const TriRec* tris, uint32_t slot, vf3& p0, vf3& p1, vf3& p2, uint32_t& prim,
-                                         uint32_t& rank, uint32_t& flags) {
-    const float4* q = (const float4*)(tris + slot);
-    float4 a = q[0], b = q[1], c = q[2];
+// Geometry of a triangle record: two 16-byte loads + one 4-byte load, all
+// float.  The integer words (prim = 9, rank = 10, flags = 11) are read with
+// their own scalar loads, only when a hit needs them.
+VR_HD void load_tri_geom(const TriRec* tris, uint32_t slot, vf3& p0, vf3& p1, vf3& p2) {
+    const f32x4* q = (const f32x4*)(tris + slot);
+    f32x4 a = q[0], b = q[1];
+    float c = ((const float*)(tris + slot))[8];
     p0 = mk3(a.x, a.y, a.z);
     p1 = mk3(a.w, b.x, b.y);
-    p2 = mk3(b.z, b.w, c.x);
-    prim = __float_as_uint(c.y);
-    rank = __float_as_uint(c.z);
-    flags = __float_as_uint(c.w);
+    p2 = mk3(b.z, b.w, c);
 }
+VR_HD uint32_t tri_word(const TriRec* tris, uint32_t slot, int w) { return ((const uint32_t*)(tris + slot))[w]; }
 
 // Confirms that the reference would consider this hit (see header comment).
-__device__ __forceinline__ bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t, vf3 p0, vf3 p1,
+VR_HD bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t, vf3 p0, vf3 p1,
                                                vf3 p2, uint32_t flags) {
     uint32_t surf = flags >> 1;
     if (sv.num_surfaces > 1) {
@@ -74,13 +75,18 @@ __device__ __forceinline__ bool confirm_winner(const SceneView& sv, vf3 o, vf3 d
 }
 
 // Result codes of the wide engine.
-enum : int { kWideDone = 0, kWideFallback = 1 };
+enum : int { kWideDone = 0, kWideFallback = 1, kWideOverflow = 2 };
 
 // Wide-engine query for one ray.  `stk` is this lane's LDS stack, entries
-// strided by `stride`.  any = true: stop at the first confirmed hit.
-template <int STACK>
-__device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, bool any,
-                                          uint2* stk, int stride, HitOut& out) {
+// strided by `stride`.  ANY = true: stop at the first confirmed hit with
+// t < tmax_any.  `out` is fully written on kWideDone.
+template <int STACK, bool ANY>
+VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, u32x2* stk, int stride,
+                     HitOut& out) {
+    out.t = 0.f;
+    out.u = 0.f;
+    out.v = 0.f;
+    out.prim = 0xFFFFFFFFu;
     const float kTiny = 1e-20f;
     float idx = 1.0f / (fabsf(d.x) > kTiny ? d.x : copysignf(kTiny, d.x));
     float idy = 1.0f / (fabsf(d.y) > kTiny ? d.y : copysignf(kTiny, d.y));
@@ -88,7 +94,7 @@ __device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, flo
     float tbest = kInf;
     uint32_t best_rank = 0, best_slot = 0xFFFFFFFFu;
     float bu = 0.f, bv = 0.f;
-    bool unconfirmed = false;
+    bool unconfirmed = false, found = false;
     int sp = 0;
     uint32_t cur = 0;
     bool have = sv.num_qnodes > 0;
@@ -96,8 +102,8 @@ __device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, flo
         if (!have) {
             if (sp == 0) break;
             --sp;
-            uint2 e = stk[sp * stride];
-            if (__uint_as_float(e.y) > tbest * kCullSlack) continue;
+            u32x2 e = stk[sp * stride];
+            if (!ANY && u2f(e.y) > tbest * kCullSlack) continue;
             cur = e.x;
         }
         have = false;
@@ -106,22 +112,26 @@ __device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, flo
             uint32_t cnt = ((cur >> 27) & 7u) + 1u;
             for (uint32_t k = 0; k < cnt; ++k) {
                 vf3 p0, p1, p2;
-                uint32_t prim, rank, flags;
-                load_tri(sv.tris, first + k, p0, p1, p2, prim, rank, flags);
+                load_tri_geom(sv.tris, first + k, p0, p1, p2);
                 MTResult r;
                 if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
-                if (any) {
-                    if (r.t < tmax_any && confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, flags)) {
-                        out.t = r.t;
-                        out.u = r.u;
-                        out.v = r.v;
-                        out.prim = prim;
-                        return kWideDone;
+                if (ANY) {
+                    if (r.t < tmax_any) {
+                        if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
+                            found = true;
+                            tbest = r.t;
+                            best_slot = first + k;
+                            bu = r.u;
+                            bv = r.v;
+                            break;
+                        }
+                        unconfirmed = true;
                     }
-                    unconfirmed = true;
                     continue;
                 }
-                if (r.t < tbest || (r.t == tbest && rank > best_rank)) {
+                if (r.t > tbest) continue;
+                uint32_t rank = tri_word(sv.tris, first + k, 10);
+                if (r.t < tbest || rank > best_rank) {
                     tbest = r.t;
                     best_rank = rank;
                     best_slot = first + k;
@@ -129,31 +139,28 @@ __device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, flo
                     bv = r.v;
                 }
             }
+            if (ANY && found) break;
             continue;
         }
-        const float4* qp = (const float4*)(sv.qnodes + cur);
-        float4 w0 = qp[0], w1 = qp[1], w2 = qp[2];
-        uint32_t e = __float_as_uint(w0.w);
-        float sx = __uint_as_float((e & 0xFFu) << 23) * idx;
-        float sy = __uint_as_float(((e >> 8) & 0xFFu) << 23) * idy;
-        float sz = __uint_as_float(((e >> 16) & 0xFFu) << 23) * idz;
-        float bx = (w0.x - o.x) * idx;
-        float by = (w0.y - o.y) * idy;
-        float bz = (w0.z - o.z) * idz;
-        uint32_t ch[4] = {__float_as_uint(w1.x), __float_as_uint(w1.y), __float_as_uint(w1.z), __float_as_uint(w1.w)};
-        uint32_t ql[3] = {__float_as_uint(w2.x), __float_as_uint(w2.y), __float_as_uint(w2.z)};
-        uint32_t qh[3] = {__float_as_uint(w2.w), 0u, 0u};
-        float4 w3 = qp[3];
-        qh[1] = __float_as_uint(w3.x);
-        qh[2] = __float_as_uint(w3.y);
-        float tcull = any ? tmax_any : tbest * kCullSlack;
+        const u32x4* qp = (const u32x4*)(sv.qnodes + cur);
+        u32x4 w0 = qp[0], w1 = qp[1], w2 = qp[2], w3 = qp[3];
+        uint32_t e = w0.w;
+        float sx = u2f((e & 0xFFu) << 23) * idx;
+        float sy = u2f(((e >> 8) & 0xFFu) << 23) * idy;
+        float sz = u2f(((e >> 16) & 0xFFu) << 23) * idz;
+        float bx = (u2f(w0.x) - o.x) * idx;
+        float by = (u2f(w0.y) - o.y) * idy;
+        float bz = (u2f(w0.z) - o.z) * idz;
+        uint32_t ch[4] = {w1.x, w1.y, w1.z, w1.w};
+        uint32_t qlx = w2.x, qly = w2.y, qlz = w2.z, qhx = w2.w, qhy = w3.x, qhz = w3.y;
+        float tcull = ANY ? tmax_any : tbest * kCullSlack;
         float tn[4];
         uint32_t cid[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            float lx = (float)((ql[0] >> (8 * c)) & 0xFFu), hx = (float)((qh[0] >> (8 * c)) & 0xFFu);
-            float ly = (float)((ql[1] >> (8 * c)) & 0xFFu), hy = (float)((qh[1] >> (8 * c)) & 0xFFu);
-            float lz = (float)((ql[2] >> (8 * c)) & 0xFFu), hz = (float)((qh[2] >> (8 * c)) & 0xFFu);
+            float lx = (float)((qlx >> (8 * c)) & 0xFFu), hx = (float)((qhx >> (8 * c)) & 0xFFu);
+            float ly = (float)((qly >> (8 * c)) & 0xFFu), hy = (float)((qhy >> (8 * c)) & 0xFFu);
+            float lz = (float)((qlz >> (8 * c)) & 0xFFu), hz = (float)((qhz >> (8 * c)) & 0xFFu);
             float t0x = fmaf(lx, sx, bx), t1x = fmaf(hx, sx, bx);
             float t0y = fmaf(ly, sy, by), t1y = fmaf(hy, sy, by);
             float t0z = fmaf(lz, sz, bz), t1z = fmaf(hz, sz, bz);
@@ -185,37 +192,42 @@ __device__ __forceinline__ int trace_wide(const SceneView& sv, vf3 o, vf3 d, flo
         if (tn[0] == kInf) continue;
         cur = cid[0];
         have = true;
+        bool overflow = false;
 #pragma unroll
         for (int c = 3; c >= 1; --c) {
             if (tn[c] == kInf) continue;
-            if (sp >= STACK) return kWideFallback;  // overflow: the exact engine answers
-            stk[sp * stride] = make_uint2(cid[c], __float_as_uint(tn[c]));
+            if (sp >= STACK) {
+                overflow = true;
+                continue;
+            }
+            stk[sp * stride] = u32x2{cid[c], f2u(tn[c])};
             ++sp;
         }
+        if (overflow) return kWideOverflow;  // the exact engine answers this ray
     }
-    if (any) {
-        if (unconfirmed) return kWideFallback;
-        out.prim = 0xFFFFFFFFu;
-        return kWideDone;
+    if (ANY) {
+        if (found) {
+            out.t = tbest;
+            out.u = bu;
+            out.v = bv;
+            out.prim = tri_word(sv.tris, best_slot, 9);
+            return kWideDone;
+        }
+        return unconfirmed ? kWideFallback : kWideDone;
     }
-    if (best_slot == 0xFFFFFFFFu) {
-        out.prim = 0xFFFFFFFFu;
-        out.t = out.u = out.v = 0.f;
-        return kWideDone;
-    }
+    if (best_slot == 0xFFFFFFFFu) return kWideDone;
     vf3 p0, p1, p2;
-    uint32_t prim, rank, flags;
-    load_tri(sv.tris, best_slot, p0, p1, p2, prim, rank, flags);
-    if (!confirm_winner(sv, o, d, tmin, tbest, p0, p1, p2, flags)) return kWideFallback;
+    load_tri_geom(sv.tris, best_slot, p0, p1, p2);
+    if (!confirm_winner(sv, o, d, tmin, tbest, p0, p1, p2, tri_word(sv.tris, best_slot, 11))) return kWideFallback;
     out.t = tbest;
     out.u = bu;
     out.v = bv;
-    out.prim = prim;
+    out.prim = tri_word(sv.tris, best_slot, 9);
     return kWideDone;
 }
 
 // Exact reference engine for one ray (closest hit).
-__device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out) {
+__host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out) {
     out.prim = 0xFFFFFFFFu;
     out.t = out.u = out.v = 0.f;
     if (!sv.top_root_valid) return;

@@ -36,6 +36,15 @@ VR_HD vf3 normalize3(vf3 a) { return a * (1.0f / mag(a)); }
 VR_HD vf3 vminc(vf3 a, float c) { return vf3{fminf(a.x, c), fminf(a.y, c), fminf(a.z, c)}; }
 VR_HD float comp(vf3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
+// Clang native vectors for wide loads/stores (plain .x/.y/.z/.w element
+// access, no wrapper classes).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+VR_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+VR_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
 static constexpr float kPi = 3.14159265358979323846f;
 static constexpr float kInf = __builtin_huge_valf();
 
