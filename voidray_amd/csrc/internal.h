@@ -141,6 +141,9 @@ struct vr_scene {
     uint32_t* counters = nullptr;  // [0] fallback count, [1..] misc
     uint32_t* host_counters = nullptr;  // pinned mirror of counters, read after a sync
     uint32_t fallback_cap = 0;
+    u32x2* spill = nullptr;             // per-lane spill stacks of the persistent traversal
+    uint32_t spill_lanes = 0;
+    uint32_t persist_blocks[2] = {0, 0};  // resident blocks of the persistent kernels
     uint32_t last_fallbacks = 0;
     // host copies used by the render driver
     uint32_t num_prims = 0;

@@ -350,6 +350,8 @@ void free_scene(vr_scene* s) {
     if (s->fallback_list) (void)hipFree(s->fallback_list);
     if (s->counters) (void)hipFree(s->counters);
     if (s->render_scratch) (void)hipFree(s->render_scratch);
+    if (s->spill) (void)hipFree(s->spill);
+    s->spill = nullptr;
     s->fallback_list = nullptr;
     s->counters = nullptr;
     s->render_scratch = nullptr;

@@ -5,8 +5,10 @@
 
 namespace vr {
 
-constexpr int kTraceBlock = 128;  // two wave64s
-constexpr int kTraceStack = 32;   // LDS stack entries per lane (8 B each)
+constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
+constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave
+constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
+constexpr int kRefill = 16;      // idle lanes that trigger a wave-wide refill
 
 struct TraceArgs {
     SceneView sv;
@@ -22,6 +24,9 @@ struct TraceArgs {
     uint8_t* hit;
     uint32_t* fb_list;
     uint32_t* fb_count;
+    uint32_t* next;       // ray counter for dynamic fetch
+    u32x2* spill;         // [kSpillCap][spill_lanes]
+    uint32_t spill_lanes;
 };
 
 __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const HitOut& h) {
@@ -32,21 +37,55 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
     if (a.hit) a.hit[i] = h.prim != 0xFFFFFFFFu ? 1 : 0;
 }
 
+// Persistent wide-engine kernel: every lane runs its own ray as a state
+// machine, one node or leaf per iteration; whenever kRefill lanes of the wave
+// are idle the wave fetches that many new rays with one atomic (dynamic ray
+// fetch), so long incoherent rays do not leave the rest of the wave idle.
 template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
-    __shared__ u32x2 stk[kTraceStack * kTraceBlock];
-    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= a.n) return;
-    vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
-    vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
-    HitOut h;
-    int rc = trace_wide<kTraceStack, ANY>(a.sv, o, d, a.tmin, kInf, stk + threadIdx.x, kTraceBlock, h);
-    if (rc != kWideDone) {
-        uint32_t slot = atomicAdd(a.fb_count, 1u);
-        a.fb_list[slot] = i;
-        return;
+__global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
+    __shared__ u32x2 lds[kLdsStack * kWave];
+    const int lane = threadIdx.x;
+    const uint32_t glane = blockIdx.x * kWave + lane;
+    const WideStack ws{lds + lane, kWave, a.spill + glane, (int)a.spill_lanes, kSpillCap};
+    const uint32_t kNone = 0xFFFFFFFFu;
+    WideState st;
+    uint32_t ray = kNone;
+    bool exhausted = false;
+    for (;;) {
+        bool idle = ray == kNone;
+        uint64_t idle_mask = __ballot(idle);
+        int nidle = __popcll(idle_mask);
+        if (!exhausted && nidle >= kRefill) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(a.next, (uint32_t)nidle);
+            base = __shfl(base, 0);
+            if (base + (uint32_t)nidle >= a.n) exhausted = true;
+            if (idle) {
+                uint32_t r = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (r < a.n) {
+                    ray = r;
+                    vf3 o = mk3(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2]);
+                    vf3 d = mk3(a.d[3 * r], a.d[3 * r + 1], a.d[3 * r + 2]);
+                    wide_init(st, o, d, a.sv.num_qnodes > 0);
+                }
+            }
+        }
+        if (exhausted && __ballot(ray != kNone) == 0ull) break;
+        if (ray != kNone) {
+            int rc = wide_step<kLdsStack, ANY>(a.sv, st, ws, a.tmin, kInf);
+            if (rc != kStepContinue) {
+                HitOut h;
+                if (rc == kStepDone) rc = wide_finish<ANY>(a.sv, st, a.tmin, h);
+                if (rc == kStepFallback) {
+                    uint32_t slot = atomicAdd(a.fb_count, 1u);
+                    a.fb_list[slot] = ray;
+                } else {
+                    write_out(a, ray, h);
+                }
+                ray = kNone;
+            }
+        }
     }
-    write_out(a, i, h);
 }
 
 // Exact engine over a ray list (fallbacks) or over all rays (list == nullptr).
@@ -62,23 +101,195 @@ __global__ __launch_bounds__(64) void k_trace_exact(TraceArgs a, const uint32_t*
     }
 }
 
+// Wave-cooperative exact engine: one wave per listed ray.  The reference
+// traversal visits every node whose ancestors all pass the slab test, so the
+// candidate set does not depend on visiting order; the wave pops up to 64
+// pending nodes at once from an LDS stack, tests them in parallel and pushes
+// the children of passing nodes.  The winner is the lexicographic minimum of
+// (t, -rank) over the wave, identical to trace_exact.  A ray whose frontier
+// outgrows the LDS stack is re-queued for the serial exact kernel.
+constexpr int kCoopStack = 6144;
+enum : uint32_t { kCoopTop = 0u, kCoopMesh = 1u, kCoopTri = 2u };
+
+__global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const uint32_t* list, const uint32_t* count,
+                                                            uint32_t* serial_list, uint32_t* serial_count) {
+    __shared__ u32x2 stk[kCoopStack];
+    const int lane = threadIdx.x;
+    const SceneView& sv = a.sv;
+    uint32_t n = list ? *count : a.n;
+    for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
+        uint32_t ray = list ? list[j] : j;
+        vf3 o = mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]);
+        vf3 d = mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]);
+        float bt = kInf, bu = 0.f, bv = 0.f;
+        uint32_t brank = 0, bprim = 0xFFFFFFFFu;
+        int sp = sv.top_root_valid ? 1 : 0;  // wave-uniform
+        if (lane == 0) stk[0] = u32x2{kCoopTop << 30, 0u};
+        bool overflow = false;
+        __syncthreads();
+        while (sp > 0) {
+            int take = sp < kWave ? sp : kWave;
+            u32x2 e = u32x2{0u, 0u};
+            bool act = lane < take;
+            if (act) e = stk[sp - take + lane];
+            __syncthreads();
+            sp -= take;
+            // children to push (up to 4: two tree children or up to four small-mesh triangles)
+            u32x2 out[4];
+            int np = 0;
+            if (act) {
+                uint32_t kind = e.x >> 30, idx = e.x & 0x3FFFFFFFu, surf = e.y;
+                if (kind == kCoopTop) {
+                    RefNode nd = sv.top_nodes[idx];
+                    if (nd.obj >= 0) {
+                        uint32_t s2 = (uint32_t)nd.obj;
+                        MeshInfo mi = sv.meshes[sv.surfaces[s2].mesh];
+                        if (mi.root >= 0) {
+                            out[np++] = u32x2{(kCoopMesh << 30) | (uint32_t)mi.root, s2};
+                        } else {
+                            for (uint32_t k = 0; k < mi.num_tris && k < 4; ++k) out[np++] = u32x2{(kCoopTri << 30) | k, s2};
+                        }
+                    } else {
+                        Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
+                        if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
+                            out[np++] = u32x2{(kCoopTop << 30) | (uint32_t)nd.right, 0u};
+                            out[np++] = u32x2{(kCoopTop << 30) | (idx + 1), 0u};
+                        }
+                    }
+                } else {
+                    SurfaceInfo si = sv.surfaces[surf];
+                    MeshInfo mi = sv.meshes[si.mesh];
+                    int32_t tri = -1;
+                    if (kind == kCoopMesh) {
+                        RefNode nd = sv.ref_nodes[idx];
+                        if (nd.obj >= 0) {
+                            tri = nd.obj;
+                        } else {
+                            Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
+                            if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
+                                out[np++] = u32x2{(kCoopMesh << 30) | (uint32_t)nd.right, surf};
+                                out[np++] = u32x2{(kCoopMesh << 30) | (idx + 1), surf};
+                            }
+                        }
+                    } else {
+                        tri = (int32_t)idx;
+                    }
+                    if (tri >= 0) {
+                        const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + (uint32_t)tri);
+                        const float* vp = sv.vpos + 3 * (size_t)mi.vert_base;
+                        vf3 p0 = mk3(vp[3 * ix[0]], vp[3 * ix[0] + 1], vp[3 * ix[0] + 2]);
+                        vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
+                        vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
+                        MTResult r;
+                        if (mt_hit(o, d, p0, p1, p2, a.tmin, r)) {
+                            uint32_t rank = si.rank_base + sv.mesh_rank[mi.tri_base + (uint32_t)tri];
+                            if (r.t < bt || (r.t == bt && rank > brank)) {
+                                bt = r.t;
+                                brank = rank;
+                                bu = r.u;
+                                bv = r.v;
+                                bprim = si.prim_base + (uint32_t)tri;
+                            }
+                        }
+                    }
+                }
+            }
+            // wave exclusive scan of push counts
+            int incl = np;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                int y = __shfl_up(incl, off);
+                if (lane >= off) incl += y;
+            }
+            int total = __shfl(incl, kWave - 1);
+            int base = sp + incl - np;
+            if (sp + total > kCoopStack) {
+                overflow = true;
+                break;
+            }
+            for (int k = 0; k < np; ++k) stk[base + k] = out[k];
+            sp += total;
+            __syncthreads();
+        }
+        __syncthreads();
+        if (overflow) {
+            if (lane == 0) {
+                uint32_t slot = atomicAdd(serial_count, 1u);
+                serial_list[slot] = ray;
+            }
+            continue;
+        }
+        // wave argmin of (t, -rank)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            float ot = __shfl_xor(bt, off);
+            uint32_t orank = __shfl_xor(brank, off);
+            float ou = __shfl_xor(bu, off), ov = __shfl_xor(bv, off);
+            uint32_t oprim = __shfl_xor(bprim, off);
+            bool take_other = oprim != 0xFFFFFFFFu &&
+                              (bprim == 0xFFFFFFFFu || ot < bt || (ot == bt && orank > brank));
+            if (take_other) {
+                bt = ot;
+                brank = orank;
+                bu = ou;
+                bv = ov;
+                bprim = oprim;
+            }
+        }
+        if (lane == 0) {
+            HitOut h;
+            h.prim = bprim;
+            h.t = bprim == 0xFFFFFFFFu ? 0.f : bt;
+            h.u = bprim == 0xFFFFFFFFu ? 0.f : bu;
+            h.v = bprim == 0xFFFFFFFFu ? 0.f : bv;
+            write_out(a, ray, h);
+        }
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
 
+static int persist_grid(vr_scene* s, bool any, uint32_t& blocks) {
+    if (s->persist_blocks[any ? 1 : 0] == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        VR_HIP_CHECK(hipGetDevice(&dev));
+        VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (any)
+            VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persist<true>, kWave, 0));
+        else
+            VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persist<false>, kWave, 0));
+        s->persist_blocks[any ? 1 : 0] = (uint32_t)std::max(1, cus * std::max(1, per_cu));
+    }
+    blocks = s->persist_blocks[any ? 1 : 0];
+    return 0;
+}
+
 int ensure_trace_scratch(vr_scene* s, uint32_t n) {
+    if (!s->spill) {
+        uint32_t b0 = 0, b1 = 0;
+        int rc = persist_grid(s, false, b0);
+        if (rc) return rc;
+        rc = persist_grid(s, true, b1);
+        if (rc) return rc;
+        s->spill_lanes = std::max(b0, b1) * kWave;
+        size_t bytes = (size_t)kSpillCap * s->spill_lanes * sizeof(u32x2);
+        if (hipMalloc((void**)&s->spill, bytes) != hipSuccess) return fail(VR_ERR_OOM, "spill stack allocation failed");
+    }
     if (!s->counters) {
         VR_HIP_CHECK(hipMalloc((void**)&s->counters, 64 * sizeof(uint32_t)));
         VR_HIP_CHECK(hipHostMalloc((void**)&s->host_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
         for (int k = 0; k < 64; ++k) s->host_counters[k] = 0;
     }
-    if (s->fallback_cap < n) {
+    // two lists of n entries: wide-engine fallbacks, then the serial-exact overflow list
+    if (s->fallback_cap < 2 * n) {
         if (s->fallback_list) VR_HIP_CHECK(hipFree(s->fallback_list));
         s->fallback_list = nullptr;
-        if (hipMalloc((void**)&s->fallback_list, (size_t)n * sizeof(uint32_t)) != hipSuccess)
+        if (hipMalloc((void**)&s->fallback_list, 2 * (size_t)n * sizeof(uint32_t)) != hipSuccess)
             return fail(VR_ERR_OOM, "fallback list allocation failed");
-        s->fallback_cap = n;
+        s->fallback_cap = 2 * n;
     }
     return 0;
 }
@@ -94,21 +305,29 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     int rc = ensure_trace_scratch(s, n);
     if (rc) return rc;
     TraceArgs a{s->view, d_o, d_d, n, desc->t_min, desc->mode == VR_TRACE_ANY ? 1 : 0, d_prim, d_t, d_u, d_v, d_hit,
-                s->fallback_list, s->counters};
+                s->fallback_list, s->counters, s->counters + 1, s->spill, s->spill_lanes};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, s->counters, 64u);
     if (desc->engine == VR_ENGINE_WIDE) {
-        dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
+        uint32_t blocks = 0;
+        rc = persist_grid(s, a.any != 0, blocks);
+        if (rc) return rc;
+        blocks = std::min<uint32_t>(blocks, (n + kWave - 1) / kWave);
         if (a.any)
-            hipLaunchKernelGGL(k_trace_wide<true>, grid, dim3(kTraceBlock), 0, stream, a);
+            hipLaunchKernelGGL(k_trace_persist<true>, dim3(blocks), dim3(kWave), 0, stream, a);
         else
-            hipLaunchKernelGGL(k_trace_wide<false>, grid, dim3(kTraceBlock), 0, stream, a);
-        // fallbacks: persistent grid reads the device-side count (no host sync)
-        hipLaunchKernelGGL(k_trace_exact, dim3(1024), dim3(64), 0, stream, a, (const uint32_t*)s->fallback_list,
-                           (const uint32_t*)s->counters);
+            hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
+        // fallbacks: one wave per ray, grid reads the device-side count (no host sync);
+        // rays whose frontier outgrows LDS go to the serial exact kernel
+        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)s->fallback_list,
+                           (const uint32_t*)s->counters, s->fallback_list + s->fallback_cap / 2, s->counters + 2);
+        hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a,
+                           (const uint32_t*)(s->fallback_list + s->fallback_cap / 2), (const uint32_t*)(s->counters + 2));
     } else {
-        uint32_t blocks = std::min<uint32_t>((n + 63) / 64, 65535u);
-        hipLaunchKernelGGL(k_trace_exact, dim3(blocks), dim3(64), 0, stream, a, (const uint32_t*)nullptr,
-                           (const uint32_t*)nullptr);
+        uint32_t blocks = std::min<uint32_t>(n, 65535u);
+        hipLaunchKernelGGL(k_trace_exact_wave, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr,
+                           (const uint32_t*)nullptr, s->fallback_list + s->fallback_cap / 2, s->counters + 2);
+        hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a,
+                           (const uint32_t*)(s->fallback_list + s->fallback_cap / 2), (const uint32_t*)(s->counters + 2));
     }
     VR_HIP_CHECK(hipGetLastError());
     VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

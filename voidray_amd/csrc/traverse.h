@@ -74,106 +74,155 @@ VR_HD bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t
     return true;
 }
 
-// Result codes of the wide engine.
-enum : int { kWideDone = 0, kWideFallback = 1, kWideOverflow = 2 };
 
-// Wide-engine query for one ray.  `stk` is this lane's LDS stack, entries
-// strided by `stride`.  ANY = true: stop at the first confirmed hit with
-// t < tmax_any.  `out` is fully written on kWideDone.
-template <int STACK, bool ANY>
-VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, u32x2* stk, int stride,
-                     HitOut& out) {
-    out.t = 0.f;
-    out.u = 0.f;
-    out.v = 0.f;
-    out.prim = 0xFFFFFFFFu;
+// ---- wide engine as a resumable per-lane state machine ----------------------
+// The stack lives in LDS (LDS_STACK entries per lane, strided by `stride`) and
+// spills its oldest entries to a per-lane global region (`spill`, strided by
+// `spill_stride`, `spill_cap` entries) when it overflows.  Each entry is
+// (node or leaf code, bits of the entry distance) so a popped subtree that
+// lies beyond the current best hit is skipped without touching its node.
+struct WideState {
+    vf3 o, d;
+    float idx, idy, idz;
+    float tbest, bu, bv;
+    uint32_t best_rank, best_slot;
+    uint32_t cur;
+    int sp, lo;
+    bool have, unconfirmed, found;
+};
+
+struct WideStack {
+    u32x2* lds;
+    int stride;
+    u32x2* spill;
+    int spill_stride, spill_cap;
+};
+
+enum : int { kStepContinue = 0, kStepDone = 1, kStepFallback = 2 };
+
+VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     const float kTiny = 1e-20f;
-    float idx = 1.0f / (fabsf(d.x) > kTiny ? d.x : copysignf(kTiny, d.x));
-    float idy = 1.0f / (fabsf(d.y) > kTiny ? d.y : copysignf(kTiny, d.y));
-    float idz = 1.0f / (fabsf(d.z) > kTiny ? d.z : copysignf(kTiny, d.z));
-    float tbest = kInf;
-    uint32_t best_rank = 0, best_slot = 0xFFFFFFFFu;
-    float bu = 0.f, bv = 0.f;
-    bool unconfirmed = false, found = false;
-    int sp = 0;
-    uint32_t cur = 0;
-    bool have = sv.num_qnodes > 0;
-    for (;;) {
-        if (!have) {
-            if (sp == 0) break;
-            --sp;
-            u32x2 e = stk[sp * stride];
-            if (!ANY && u2f(e.y) > tbest * kCullSlack) continue;
-            cur = e.x;
+    st.o = o;
+    st.d = d;
+    st.idx = 1.0f / (fabsf(d.x) > kTiny ? d.x : copysignf(kTiny, d.x));
+    st.idy = 1.0f / (fabsf(d.y) > kTiny ? d.y : copysignf(kTiny, d.y));
+    st.idz = 1.0f / (fabsf(d.z) > kTiny ? d.z : copysignf(kTiny, d.z));
+    st.tbest = kInf;
+    st.bu = st.bv = 0.f;
+    st.best_rank = 0;
+    st.best_slot = 0xFFFFFFFFu;
+    st.cur = 0;
+    st.sp = 0;
+    st.lo = 0;
+    st.have = nonempty;
+    st.unconfirmed = false;
+    st.found = false;
+}
+
+template <int LDS_STACK>
+VR_HD bool wide_push(WideState& st, const WideStack& ws, u32x2 e) {
+    if (st.sp - st.lo == LDS_STACK) {
+        if (st.lo >= ws.spill_cap) return false;
+        ws.spill[st.lo * ws.spill_stride] = ws.lds[(st.lo & (LDS_STACK - 1)) * ws.stride];
+        ++st.lo;
+    }
+    ws.lds[(st.sp & (LDS_STACK - 1)) * ws.stride] = e;
+    ++st.sp;
+    return true;
+}
+
+template <int LDS_STACK>
+VR_HD u32x2 wide_pop(WideState& st, const WideStack& ws) {
+    --st.sp;
+    if (st.sp < st.lo) {
+        st.lo = st.sp;
+        return ws.spill[st.sp * ws.spill_stride];
+    }
+    return ws.lds[(st.sp & (LDS_STACK - 1)) * ws.stride];
+}
+
+// One traversal step (one node or one leaf).  Returns kStepContinue,
+// kStepDone (result ready for wide_finish) or kStepFallback.
+template <int LDS_STACK, bool ANY>
+VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
+    if (!st.have) {
+        for (;;) {
+            if (st.sp == 0) return kStepDone;
+            u32x2 e = wide_pop<LDS_STACK>(st, ws);
+            if (!ANY && u2f(e.y) > st.tbest * kCullSlack) continue;
+            st.cur = e.x;
+            break;
         }
-        have = false;
-        if (cur & kLeafBit) {
-            uint32_t first = cur & 0x07FFFFFFu;
-            uint32_t cnt = ((cur >> 27) & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                vf3 p0, p1, p2;
-                load_tri_geom(sv.tris, first + k, p0, p1, p2);
-                MTResult r;
-                if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
-                if (ANY) {
-                    if (r.t < tmax_any) {
-                        if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
-                            found = true;
-                            tbest = r.t;
-                            best_slot = first + k;
-                            bu = r.u;
-                            bv = r.v;
-                            break;
-                        }
-                        unconfirmed = true;
+    }
+    st.have = false;
+    const vf3 o = st.o, d = st.d;
+    uint32_t cur = st.cur;
+    if (cur & kLeafBit) {
+        uint32_t first = cur & 0x07FFFFFFu;
+        uint32_t cnt = ((cur >> 27) & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            vf3 p0, p1, p2;
+            load_tri_geom(sv.tris, first + k, p0, p1, p2);
+            MTResult r;
+            if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
+            if (ANY) {
+                if (r.t < tmax_any) {
+                    if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
+                        st.found = true;
+                        st.tbest = r.t;
+                        st.best_slot = first + k;
+                        st.bu = r.u;
+                        st.bv = r.v;
+                        return kStepDone;
                     }
-                    continue;
+                    st.unconfirmed = true;
                 }
-                if (r.t > tbest) continue;
-                uint32_t rank = tri_word(sv.tris, first + k, 10);
-                if (r.t < tbest || rank > best_rank) {
-                    tbest = r.t;
-                    best_rank = rank;
-                    best_slot = first + k;
-                    bu = r.u;
-                    bv = r.v;
-                }
+                continue;
             }
-            if (ANY && found) break;
-            continue;
+            if (r.t > st.tbest) continue;
+            uint32_t rank = tri_word(sv.tris, first + k, 10);
+            if (r.t < st.tbest || rank > st.best_rank) {
+                st.tbest = r.t;
+                st.best_rank = rank;
+                st.best_slot = first + k;
+                st.bu = r.u;
+                st.bv = r.v;
+            }
         }
-        const u32x4* qp = (const u32x4*)(sv.qnodes + cur);
-        u32x4 w0 = qp[0], w1 = qp[1], w2 = qp[2], w3 = qp[3];
-        uint32_t e = w0.w;
-        float sx = u2f((e & 0xFFu) << 23) * idx;
-        float sy = u2f(((e >> 8) & 0xFFu) << 23) * idy;
-        float sz = u2f(((e >> 16) & 0xFFu) << 23) * idz;
-        float bx = (u2f(w0.x) - o.x) * idx;
-        float by = (u2f(w0.y) - o.y) * idy;
-        float bz = (u2f(w0.z) - o.z) * idz;
-        uint32_t ch[4] = {w1.x, w1.y, w1.z, w1.w};
-        uint32_t qlx = w2.x, qly = w2.y, qlz = w2.z, qhx = w2.w, qhy = w3.x, qhz = w3.y;
-        float tcull = ANY ? tmax_any : tbest * kCullSlack;
-        float tn[4];
-        uint32_t cid[4];
+        return kStepContinue;
+    }
+    const u32x4* qp = (const u32x4*)(sv.qnodes + cur);
+    u32x4 w0 = qp[0], w1 = qp[1], w2 = qp[2], w3 = qp[3];
+    uint32_t e = w0.w;
+    float sx = u2f((e & 0xFFu) << 23) * st.idx;
+    float sy = u2f(((e >> 8) & 0xFFu) << 23) * st.idy;
+    float sz = u2f(((e >> 16) & 0xFFu) << 23) * st.idz;
+    float bx = (u2f(w0.x) - o.x) * st.idx;
+    float by = (u2f(w0.y) - o.y) * st.idy;
+    float bz = (u2f(w0.z) - o.z) * st.idz;
+    uint32_t ch[4] = {w1.x, w1.y, w1.z, w1.w};
+    uint32_t qlx = w2.x, qly = w2.y, qlz = w2.z, qhx = w2.w, qhy = w3.x, qhz = w3.y;
+    float tcull = ANY ? tmax_any : st.tbest * kCullSlack;
+    float tn[4];
+    uint32_t cid[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float lx = (float)((qlx >> (8 * c)) & 0xFFu), hx = (float)((qhx >> (8 * c)) & 0xFFu);
-            float ly = (float)((qly >> (8 * c)) & 0xFFu), hy = (float)((qhy >> (8 * c)) & 0xFFu);
-            float lz = (float)((qlz >> (8 * c)) & 0xFFu), hz = (float)((qhz >> (8 * c)) & 0xFFu);
-            float t0x = fmaf(lx, sx, bx), t1x = fmaf(hx, sx, bx);
-            float t0y = fmaf(ly, sy, by), t1y = fmaf(hy, sy, by);
-            float t0z = fmaf(lz, sz, bz), t1z = fmaf(hz, sz, bz);
-            float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
-            float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
-            float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
-            float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
-            float tfar = fminf(fminf(fx, fy), fminf(fz, tcull));
-            bool hit = ch[c] != kInvalidChild && tnear <= tfar * kSlabSlack;
-            tn[c] = hit ? tnear : kInf;
-            cid[c] = ch[c];
-        }
-        // sort (tnear, child) ascending; misses carry +inf and sink to the end
+    for (int c = 0; c < 4; ++c) {
+        float lx = (float)((qlx >> (8 * c)) & 0xFFu), hx = (float)((qhx >> (8 * c)) & 0xFFu);
+        float ly = (float)((qly >> (8 * c)) & 0xFFu), hy = (float)((qhy >> (8 * c)) & 0xFFu);
+        float lz = (float)((qlz >> (8 * c)) & 0xFFu), hz = (float)((qhz >> (8 * c)) & 0xFFu);
+        float t0x = fmaf(lx, sx, bx), t1x = fmaf(hx, sx, bx);
+        float t0y = fmaf(ly, sy, by), t1y = fmaf(hy, sy, by);
+        float t0z = fmaf(lz, sz, bz), t1z = fmaf(hz, sz, bz);
+        float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
+        float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
+        float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
+        float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+        float tfar = fminf(fminf(fx, fy), fminf(fz, tcull));
+        bool hit = ch[c] != kInvalidChild && tnear <= tfar * kSlabSlack;
+        tn[c] = hit ? tnear : kInf;
+        cid[c] = ch[c];
+    }
+    // sort (tnear, child) ascending; misses carry +inf and sink to the end
 #define VR_CSWAP(a, b)                                   \
     if (tn[b] < tn[a]) {                                 \
         float tt = tn[a];                                \
@@ -183,47 +232,68 @@ VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_a
         cid[a] = cid[b];                                 \
         cid[b] = cc;                                     \
     }
-        VR_CSWAP(0, 1)
-        VR_CSWAP(2, 3)
-        VR_CSWAP(0, 2)
-        VR_CSWAP(1, 3)
-        VR_CSWAP(1, 2)
+    VR_CSWAP(0, 1)
+    VR_CSWAP(2, 3)
+    VR_CSWAP(0, 2)
+    VR_CSWAP(1, 3)
+    VR_CSWAP(1, 2)
 #undef VR_CSWAP
-        if (tn[0] == kInf) continue;
-        cur = cid[0];
-        have = true;
-        bool overflow = false;
+    if (tn[0] == kInf) return kStepContinue;
+    st.cur = cid[0];
+    st.have = true;
+    bool ok = true;
 #pragma unroll
-        for (int c = 3; c >= 1; --c) {
-            if (tn[c] == kInf) continue;
-            if (sp >= STACK) {
-                overflow = true;
-                continue;
-            }
-            stk[sp * stride] = u32x2{cid[c], f2u(tn[c])};
-            ++sp;
-        }
-        if (overflow) return kWideOverflow;  // the exact engine answers this ray
+    for (int c = 3; c >= 1; --c) {
+        if (tn[c] == kInf) continue;
+        ok = ok && wide_push<LDS_STACK>(st, ws, u32x2{cid[c], f2u(tn[c])});
     }
+    return ok ? kStepContinue : kStepFallback;  // stack exhausted: the exact engine answers
+}
+
+// Result of a finished query (after kStepDone).  Returns kStepDone or
+// kStepFallback (winner not confirmed / any-hit candidates unconfirmed).
+template <bool ANY>
+VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitOut& out) {
+    out.t = 0.f;
+    out.u = 0.f;
+    out.v = 0.f;
+    out.prim = 0xFFFFFFFFu;
     if (ANY) {
-        if (found) {
-            out.t = tbest;
-            out.u = bu;
-            out.v = bv;
-            out.prim = tri_word(sv.tris, best_slot, 9);
-            return kWideDone;
+        if (st.found) {
+            out.t = st.tbest;
+            out.u = st.bu;
+            out.v = st.bv;
+            out.prim = tri_word(sv.tris, st.best_slot, 9);
+            return kStepDone;
         }
-        return unconfirmed ? kWideFallback : kWideDone;
+        return st.unconfirmed ? kStepFallback : kStepDone;
     }
-    if (best_slot == 0xFFFFFFFFu) return kWideDone;
+    if (st.best_slot == 0xFFFFFFFFu) return kStepDone;
     vf3 p0, p1, p2;
-    load_tri_geom(sv.tris, best_slot, p0, p1, p2);
-    if (!confirm_winner(sv, o, d, tmin, tbest, p0, p1, p2, tri_word(sv.tris, best_slot, 11))) return kWideFallback;
-    out.t = tbest;
-    out.u = bu;
-    out.v = bv;
-    out.prim = tri_word(sv.tris, best_slot, 9);
-    return kWideDone;
+    load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
+    if (!confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11)))
+        return kStepFallback;
+    out.t = st.tbest;
+    out.u = st.bu;
+    out.v = st.bv;
+    out.prim = tri_word(sv.tris, st.best_slot, 9);
+    return kStepDone;
+}
+
+// Whole query for one ray (used by the host-side checks and simple kernels).
+template <int LDS_STACK, bool ANY>
+VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, const WideStack& ws,
+                     HitOut& out) {
+    WideState st;
+    wide_init(st, o, d, sv.num_qnodes > 0);
+    int rc;
+    while ((rc = wide_step<LDS_STACK, ANY>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
+    }
+    if (rc == kStepFallback) {
+        out.prim = 0xFFFFFFFFu;
+        return kStepFallback;
+    }
+    return wide_finish<ANY>(sv, st, tmin, out);
 }
 
 // Exact reference engine for one ray (closest hit).
