@@ -227,9 +227,13 @@ typedef struct vr_render_stats {
  * width*height*4 f32, additive, like iterative.rs:45-51). */
 int vr_render(vr_scene* scene, const vr_render_desc* desc, float* rgba_accum,
               vr_render_stats* stats);
-/* Same into a device accumulator. */
+/* Same into a device accumulator, enqueued on `stream` (NULL = context
+ * stream); `stats` is not filled (use vr_render_last_stats). */
 int vr_render_device_async(vr_scene* scene, const vr_render_desc* desc,
                            float* d_rgba_accum, vr_render_stats* stats, void* stream);
+/* Counters of the last render on this scene (waits for the context stream;
+ * renders enqueued on other streams must be synchronised by the caller). */
+int vr_render_last_stats(vr_scene* scene, vr_render_stats* stats);
 
 /* ---- post-processing (shaders/post_process.glsl:23-49) ----------------- */
 

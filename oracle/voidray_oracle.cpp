@@ -663,7 +663,7 @@ template <class R> void render_pixel(const RenderJob<R>& job, uint32_t index, Re
     px[0] += color.x;
     px[1] += color.y;
     px[2] += color.z;
-    px[3] += R(1);
+    if (d.flags & VR_RENDER_ALPHA_PER_CALL) px[3] += R(1);
 }
 
 template <class R> void render_all(const RenderJob<R>& job, int nthreads, RenderCounters& total) {

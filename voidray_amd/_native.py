@@ -110,6 +110,7 @@ SIGNATURES = {
     "vr_trace_last_fallbacks": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "vr_render": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats)]),
     "vr_render_device_async": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats), _P]),
+    "vr_render_last_stats": (C.c_int, [_P, C.POINTER(vr_render_stats)]),
     "vr_tonemap": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P]),
     "vr_tonemap_device_async": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P, _P]),
 }

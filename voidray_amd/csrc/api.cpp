@@ -179,13 +179,37 @@ int vr_render(vr_scene* s, const vr_render_desc* desc, float* rgba, vr_render_st
         int rc = ensure_dev_stage(bytes);
         if (rc) return rc;
         VR_HIP_CHECK(hipMemcpyAsync(g_dev_stage.p, rgba, bytes, hipMemcpyHostToDevice, st));
+        hipEvent_t e0, e1;
+        VR_HIP_CHECK(hipEventCreate(&e0));
+        VR_HIP_CHECK(hipEventCreate(&e1));
+        VR_HIP_CHECK(hipEventRecord(e0, st));
         rc = render_device(s, desc, (float*)g_dev_stage.p, stats, st);
-        if (rc) return rc;
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        VR_HIP_CHECK(hipEventRecord(e1, st));
         VR_HIP_CHECK(hipMemcpyAsync(rgba, g_dev_stage.p, bytes, hipMemcpyDeviceToHost, st));
         VR_HIP_CHECK(hipStreamSynchronize(st));
-        if (stats) render_finish_stats(s, stats);
+        float ms = 0.f;
+        VR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (stats) {
+            stats->kernel_ms = ms;
+            render_finish_stats(s, stats);
+        }
         return 0;
     });
+}
+
+int vr_render_last_stats(vr_scene* s, vr_render_stats* stats) {
+    if (!s || !stats) return fail(VR_ERR_INVALID, "null argument");
+    VR_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    stats->kernel_ms = 0.0;
+    render_finish_stats(s, stats);
+    return 0;
 }
 
 int vr_tonemap_device_async(vr_ctx* ctx, const vr_post_desc* desc, const float* d_src, float* d_dst, void* stream) {

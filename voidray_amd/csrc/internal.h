@@ -150,6 +150,8 @@ struct vr_scene {
     // render scratch (grown on demand)
     void* render_scratch = nullptr;
     size_t render_scratch_bytes = 0;
+    uint64_t* render_stats_dev = nullptr;   // [segments, samples, misses, fallbacks, capped]
+    uint64_t* render_stats_host = nullptr;  // pinned copy, valid after the stream completes
 };
 
 namespace vr {
@@ -175,6 +177,21 @@ int build_host_scene(const vr_scene_desc* desc, HostScene& h);
 int build_scene(vr_ctx* ctx, const vr_scene_desc* desc, vr_scene* s);
 void free_scene(vr_scene* s);
 // trace.hip
+struct TraceJob {
+    const float* o = nullptr;       // 3 floats per ray
+    const float* d = nullptr;
+    uint32_t n_max = 0;             // rays (or queue capacity)
+    const uint32_t* queue = nullptr;  // optional indirection
+    const uint32_t* count = nullptr;  // optional device-side count
+    float tmin = 1e-5f, tmax = kInf;
+    bool any = false;
+    int engine = VR_ENGINE_WIDE;
+    uint32_t* prim = nullptr;
+    float *t = nullptr, *u = nullptr, *v = nullptr;
+    uint8_t* hit = nullptr;
+    uint64_t* fallback_accum = nullptr;  // optional: += number of exact-engine rays
+};
+int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream);
 int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d, uint32_t* d_prim,
                  float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream);
 int ensure_trace_scratch(vr_scene* s, uint32_t n);

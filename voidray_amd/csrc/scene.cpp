@@ -351,6 +351,10 @@ void free_scene(vr_scene* s) {
     if (s->counters) (void)hipFree(s->counters);
     if (s->render_scratch) (void)hipFree(s->render_scratch);
     if (s->spill) (void)hipFree(s->spill);
+    if (s->render_stats_dev) (void)hipFree(s->render_stats_dev);
+    if (s->render_stats_host) (void)hipHostFree(s->render_stats_host);
+    s->render_stats_dev = nullptr;
+    s->render_stats_host = nullptr;
     s->spill = nullptr;
     s->fallback_list = nullptr;
     s->counters = nullptr;

@@ -1,0 +1,305 @@
+// Shading routines of the wavefront renderer: hit records, textures, the
+// environment, random distributions and the materials.  Every f32 expression
+// follows the operation order of the reference so that, with the same random
+// words, a path makes the same decisions as voidray's CPU integrator.
+#pragma once
+#include "internal.h"
+
+namespace vr {
+
+// Rust `x as usize` (saturating; NaN -> 0).
+VR_HD uint64_t as_usize(float x) {
+    if (!(x >= 0.0f)) return 0;
+    if (x >= 18446744073709551616.0f) return ~0ull;
+    return (uint64_t)x;
+}
+
+VR_HD vf3 texel(const SceneView& sv, const GpuTexture& t, uint64_t i) {
+    if (t.format == VR_TEX_RGBA8) {
+        const uint8_t* p = sv.tex_data + t.offset + 4 * i;
+        return mk3((float)p[0] / 255.0f, (float)p[1] / 255.0f, (float)p[2] / 255.0f);
+    }
+    const float* p = (const float*)(sv.tex_data + t.offset) + 3 * i;
+    return mk3(p[0], p[1], p[2]);
+}
+
+VR_HD vf3 lerp3(vf3 a, vf3 b, float t) { return a * (1.0f - t) + b * t; }
+
+// ImageTexture::bilinear_sample (texture.rs:59-78); HDRIEnvironment uses the
+// same arithmetic (environments.rs:57-76).
+template <class Fetch>
+VR_HD vf3 bilinear(uint32_t W, uint32_t H, float x, float y, Fetch fetch) {
+    uint64_t len = (uint64_t)W * H;
+    uint64_t x0 = as_usize(x);
+    if (x0 > W - 1) x0 = W - 1;
+    uint64_t y0 = as_usize(y);
+    if (y0 > H - 1) y0 = H - 1;
+    float ax = x - (float)x0;
+    float ay = y - (float)y0;
+    return lerp3(lerp3(fetch((y0 * W + x0) % len), fetch((y0 * W + x0 + 1) % len), ax),
+                 lerp3(fetch(((y0 + 1) * W + x0) % len), fetch(((y0 + 1) * W + x0 + 1) % len), ax), ay);
+}
+
+// ImageTexture::sample (texture.rs:82-98)
+VR_HD vf3 tex_sample(const SceneView& sv, int32_t ti, float u, float v) {
+    const GpuTexture t = sv.textures[ti];
+    if (u < 0.0f) u -= truncf(u) - 1.0f;
+    if (v < 0.0f) v -= truncf(v) - 1.0f;
+    float x = fmodf(u, 1.0f) * (float)t.w;
+    float y = (1.0f - fmodf(v, 1.0f)) * (float)t.h;
+    if (t.sample_type == VR_SAMPLE_NEAREST) {
+        uint64_t xi = as_usize(x), yi = as_usize(y);
+        if (xi > t.w - 1) xi = t.w - 1;
+        if (yi > t.h - 1) yi = t.h - 1;
+        return texel(sv, t, (yi * t.w + xi) % ((uint64_t)t.w * t.h));
+    }
+    return bilinear(t.w, t.h, x, y, [&](uint64_t i) { return texel(sv, t, i); });
+}
+
+VR_HD vf3 env_texel(const SceneView& sv, uint64_t i) { return mk3(sv.env[3 * i], sv.env[3 * i + 1], sv.env[3 * i + 2]); }
+
+// Environment::sample: UniformEnvironment (environments.rs:29-33) and
+// HDRIEnvironment::sample with to_spherical_coords (environments.rs:80-90,
+// util/math.rs:24-29).
+VR_HD vf3 env_lookup(const SceneView& sv, vf3 dir) {
+    if (sv.env_kind == VR_ENV_UNIFORM) return mk3(sv.env_color[0], sv.env_color[1], sv.env_color[2]);
+    if (sv.env_kind != VR_ENV_HDRI) return mk3(0.f, 0.f, 0.f);
+    vf3 d = normalize3(dir);
+    float th = acosf(-d.y);
+    float ph = atan2f(-d.z, d.x) + kPi;
+    float u = th / kPi;
+    float v = ph / (2.0f * kPi);
+    float x = v * (float)sv.env_w;
+    float y = (float)(sv.env_h - 1) - (u * (float)sv.env_h);
+    return bilinear(sv.env_w, sv.env_h, x, y, [&](uint64_t i) { return env_texel(sv, i); });
+}
+
+// ---- random distributions (rand_distr 0.4.3 algorithms, capped) ------------
+constexpr int kRejectCap = 32;
+
+// UnitSphere: Marsaglia (1972).
+VR_HD vf3 unit_sphere(PathRng& rng, uint32_t& capped) {
+    for (int i = 0; i < kRejectCap; ++i) {
+        float x1 = rng.uniform() * 2.0f - 1.0f;
+        float x2 = rng.uniform() * 2.0f - 1.0f;
+        float sum = x1 * x1 + x2 * x2;
+        if (sum >= 1.0f) continue;
+        float factor = 2.0f * sqrtf(1.0f - sum);
+        return mk3(x1 * factor, x2 * factor, 1.0f - 2.0f * sum);
+    }
+    ++capped;
+    return mk3(0.f, 0.f, 1.f);
+}
+
+// UnitDisc: square rejection.
+VR_HD vf2 unit_disc(PathRng& rng, uint32_t& capped) {
+    for (int i = 0; i < kRejectCap; ++i) {
+        float x1 = rng.uniform() * 2.0f - 1.0f;
+        float x2 = rng.uniform() * 2.0f - 1.0f;
+        if (x1 * x1 + x2 * x2 <= 1.0f) return vf2{x1, x2};
+    }
+    ++capped;
+    return vf2{0.f, 0.f};
+}
+
+VR_HD bool near_zero(vf3 v) {
+    const float e = 1.0e-8f;
+    return fabsf(v.x) < e && fabsf(v.y) < e && fabsf(v.z) < e;
+}
+// reflect / refract (util/math.rs:12-21)
+VR_HD vf3 reflect3(vf3 v, vf3 n) { return v - (2.0f * dot3(v, n)) * n; }
+VR_HD vf3 refract3(vf3 uv, vf3 n, float eta) {
+    float cos_t = fminf(dot3(n, -uv), 1.0f);
+    vf3 perp = eta * (uv + cos_t * n);
+    vf3 par = (-sqrtf(fabsf(1.0f - mag2(perp)))) * n;
+    return perp + par;
+}
+// Dielectric::reflectance (simple.rs:192-197); powi(x, 5) = x * (x*x)^2
+VR_HD float schlick(float c, float idx) {
+    float r0 = (1.0f - idx) / (1.0f + idx);
+    r0 = r0 * r0;
+    float x = 1.0f - c;
+    float x2 = x * x;
+    return r0 + (1.0f - r0) * (x * (x2 * x2));
+}
+
+// HitRecord (util/ray.rs:27-49) for a closest hit (prim, t, u, v) of ray (o, d).
+struct HitRec {
+    vf3 point, normal;
+    vf2 uv;
+    bool front;
+    uint32_t surface;
+};
+
+VR_HD HitRec make_hit(const SceneView& sv, vf3 o, vf3 d, uint32_t prim, float t, float u, float v) {
+    HitRec h;
+    uint32_t surf = sv.prim_surface[prim];
+    SurfaceInfo si = sv.surfaces[surf];
+    MeshInfo mi = sv.meshes[si.mesh];
+    uint32_t tri = prim - si.prim_base;
+    const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + tri);
+    size_t v0 = mi.vert_base + ix[0], v1 = mi.vert_base + ix[1], v2 = mi.vert_base + ix[2];
+    vf3 n0 = mk3(sv.vnrm[3 * v0], sv.vnrm[3 * v0 + 1], sv.vnrm[3 * v0 + 2]);
+    vf3 n1 = mk3(sv.vnrm[3 * v1], sv.vnrm[3 * v1 + 1], sv.vnrm[3 * v1 + 2]);
+    vf3 n2 = mk3(sv.vnrm[3 * v2], sv.vnrm[3 * v2 + 1], sv.vnrm[3 * v2 + 2]);
+    float w = (1.0f - u) - v;
+    // Triangle::hit shading (mesh.rs:176-181)
+    vf3 n = (u * n1 + v * n2) + w * n0;
+    h.uv.x = (u * sv.vuv[2 * v1] + v * sv.vuv[2 * v2]) + w * sv.vuv[2 * v0];
+    h.uv.y = (u * sv.vuv[2 * v1 + 1] + v * sv.vuv[2 * v2 + 1]) + w * sv.vuv[2 * v0 + 1];
+    const float* fp = sv.face_n + 3 * (size_t)(mi.tri_base + tri);
+    vf3 fn = mk3(fp[0], fp[1], fp[2]);
+    const float thirty = (30.0f * kPi) / 180.0f;
+    if (atan2f(mag(cross3(n, fn)), dot3(n, fn)) > thirty) n = fn;
+    // HitRecord::new
+    h.point = o + d * t;
+    h.front = dot3(d, n) < 0.0f;
+    h.normal = h.front ? n : -n;
+    h.surface = surf;
+    return h;
+}
+
+struct ScatterOut {
+    vf3 att;
+    vf3 dir;   // normalised (Ray::new)
+    bool has_ray;
+    bool diffuse;  // Lambertian (cosine lobe about `lobe`) for MIS
+    vf3 lobe;      // unit lobe axis of a diffuse scatter
+};
+
+// Material::scatter (voidray_common/src/simple.rs, core/traits.rs:19-41).
+// `cosine_lobe`: the extended estimator's Lambertian, which scatters about the
+// normalised shading normal so that its pdf is cos/pi exactly.
+VR_HD ScatterOut scatter(const SceneView& sv, const GpuMaterial& m, vf3 d, const HitRec& hit, PathRng& rng,
+                         uint32_t& capped, bool cosine_lobe) {
+    ScatterOut s;
+    s.diffuse = false;
+    s.lobe = mk3(0.f, 0.f, 0.f);
+    switch (m.kind) {
+        case VR_MAT_LAMBERTIAN: {
+            vf3 n = m.normal_tex >= 0 ? tex_sample(sv, m.normal_tex, hit.uv.x, hit.uv.y) : hit.normal;
+            if (cosine_lobe) n = normalize3(n);
+            vf3 dir = n + unit_sphere(rng, capped);
+            if (near_zero(dir)) dir = n;
+            s.dir = normalize3(dir);
+            s.has_ray = true;
+            s.att = m.albedo_tex >= 0 ? tex_sample(sv, m.albedo_tex, hit.uv.x, hit.uv.y)
+                                      : mk3(m.color[0], m.color[1], m.color[2]);
+            s.diffuse = true;
+            s.lobe = n;
+            return s;
+        }
+        case VR_MAT_METAL: {
+            vf3 refl = normalize3(reflect3(d, hit.normal));
+            for (int i = 0; i < kRejectCap; ++i) {
+                vf3 sd = normalize3(refl + m.param * unit_sphere(rng, capped));
+                if (dot3(sd, hit.normal) > 0.0f) {
+                    s.att = mk3(m.color[0], m.color[1], m.color[2]);
+                    s.dir = sd;
+                    s.has_ray = true;
+                    return s;
+                }
+            }
+            ++capped;  // the reference loops forever here; the path is absorbed
+            s.att = mk3(0.f, 0.f, 0.f);
+            s.has_ray = false;
+            return s;
+        }
+        case VR_MAT_DIELECTRIC: {
+            float ratio = hit.front ? 1.0f / m.param : m.param;
+            vf3 ud = normalize3(d);
+            float cos_t = fminf(dot3(hit.normal, -ud), 1.0f);
+            float sin_t = sqrtf(1.0f - cos_t * cos_t);
+            bool cannot = (ratio * sin_t) > 1.0f;
+            vf3 dir;
+            if (cannot || schlick(cos_t, ratio) > rng.uniform()) dir = reflect3(ud, hit.normal);
+            else dir = refract3(ud, hit.normal, ratio);
+            s.att = mk3(1.f, 1.f, 1.f);
+            s.dir = normalize3(dir);
+            s.has_ray = true;
+            return s;
+        }
+        case VR_MAT_EMISSION:
+            s.att = mk3(m.color[0], m.color[1], m.color[2]);
+            s.has_ray = false;
+            return s;
+        case VR_MAT_LAMBERTIAN_BSDF: {
+            if (mag2(hit.normal) == 0.0f) {  // the reference asserts here
+                ++capped;
+                s.att = mk3(0.f, 0.f, 0.f);
+                s.has_ray = false;
+                return s;
+            }
+            vf3 wi = normalize3(unit_sphere(rng, capped));
+            float pdf = 1.0f;
+            vf3 f = mk3(m.color[0], m.color[1], m.color[2]) / kPi;
+            s.dir = normalize3(wi);
+            s.att = (f * fabsf(dot3(wi, hit.normal))) * (1.0f / pdf);
+            s.has_ray = true;
+            return s;
+        }
+    }
+    s.att = mk3(0.f, 0.f, 0.f);
+    s.has_ray = false;
+    return s;
+}
+
+// ---- environment importance sampling (extended estimator) -------------------
+// Continuous lookup coordinates (x, y) of HDRIEnvironment::sample map to
+// theta = pi (H - 1 - y) / H, phi = 2 pi x / W; a cell has unit area in (x, y),
+// so pdf_omega = pdf_cell * W * H / (2 pi^2 sin theta).
+VR_HD uint32_t cdf_search(const float* cdf, uint32_t n, float u) {
+    // largest k in [0, n) with cdf[k] <= u  (cdf[0] = 0, cdf[n] = 1)
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+VR_HD float env_pdf_omega(const SceneView& sv, float x, float y, float sin_t) {
+    if (!(sin_t > 0.f)) return 0.f;
+    int64_t i = (int64_t)floorf(x);
+    int64_t r = (int64_t)floorf(y) + 1;
+    if (i < 0) i = 0;
+    if (i > (int64_t)sv.env_w - 1) i = sv.env_w - 1;
+    if (r < 0) r = 0;
+    if (r > (int64_t)sv.env_h - 1) r = sv.env_h - 1;
+    float pc = sv.env_pdf[(size_t)r * sv.env_w + (size_t)i];
+    return pc * ((float)sv.env_w * (float)sv.env_h) / (2.0f * kPi * kPi * sin_t);
+}
+
+// pdf (solid angle) of sampling direction d by the environment tables.
+VR_HD float env_pdf_dir(const SceneView& sv, vf3 dir) {
+    vf3 d = normalize3(dir);
+    float th = acosf(-d.y);
+    float ph = atan2f(-d.z, d.x) + kPi;
+    float x = (ph / (2.0f * kPi)) * (float)sv.env_w;
+    float y = (float)(sv.env_h - 1) - (th / kPi) * (float)sv.env_h;
+    return env_pdf_omega(sv, x, y, sinf(th));
+}
+
+VR_HD vf3 env_sample_dir(const SceneView& sv, float u1, float u2, float& pdf) {
+    uint32_t H = sv.env_h, W = sv.env_w;
+    uint32_t r = cdf_search(sv.env_marg_cdf, H, u1);
+    float m0 = sv.env_marg_cdf[r], m1 = sv.env_marg_cdf[r + 1];
+    const float* c = sv.env_cond_cdf + (size_t)r * (W + 1);
+    uint32_t i = cdf_search(c, W, u2);
+    float c0 = c[i], c1 = c[i + 1];
+    float fx = c1 > c0 ? (u2 - c0) / (c1 - c0) : 0.5f;
+    float fy = m1 > m0 ? (u1 - m0) / (m1 - m0) : 0.5f;
+    fx = fminf(fmaxf(fx, 0.f), 0.99999994f);
+    fy = fminf(fmaxf(fy, 0.f), 0.99999994f);
+    float x = (float)i + fx;
+    float y = ((float)r - 1.0f) + fy;
+    float th = kPi * ((float)H - 1.0f - y) / (float)H;
+    float ph = 2.0f * kPi * x / (float)W;
+    float st = sinf(th), ct = cosf(th);
+    float sp = sinf(ph - kPi), cp = cosf(ph - kPi);
+    pdf = env_pdf_omega(sv, x, y, st);
+    return mk3(st * cp, -ct, -st * sp);
+}
+
+}  // namespace vr

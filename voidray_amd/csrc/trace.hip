@@ -27,6 +27,9 @@ struct TraceArgs {
     uint32_t* next;       // ray counter for dynamic fetch
     u32x2* spill;         // [kSpillCap][spill_lanes]
     uint32_t spill_lanes;
+    const uint32_t* queue;  // optional: item k traces ray queue[k] (else ray k)
+    const uint32_t* count;  // optional: device-side item count (else n)
+    float tmax;             // any-hit: hits at t >= tmax do not count
 };
 
 __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const HitOut& h) {
@@ -48,6 +51,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     const uint32_t glane = blockIdx.x * kWave + lane;
     const WideStack ws{lds + lane, kWave, a.spill + glane, (int)a.spill_lanes, kSpillCap};
     const uint32_t kNone = 0xFFFFFFFFu;
+    const uint32_t n = a.count ? *a.count : a.n;
     WideState st;
     uint32_t ray = kNone;
     bool exhausted = false;
@@ -59,10 +63,11 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(a.next, (uint32_t)nidle);
             base = __shfl(base, 0);
-            if (base + (uint32_t)nidle >= a.n) exhausted = true;
+            if (base + (uint32_t)nidle >= n) exhausted = true;
             if (idle) {
-                uint32_t r = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
-                if (r < a.n) {
+                uint32_t k = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (k < n) {
+                    uint32_t r = a.queue ? a.queue[k] : k;
                     ray = r;
                     vf3 o = mk3(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2]);
                     vf3 d = mk3(a.d[3 * r], a.d[3 * r + 1], a.d[3 * r + 2]);
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
         }
         if (exhausted && __ballot(ray != kNone) == 0ull) break;
         if (ray != kNone) {
-            int rc = wide_step<kLdsStack, ANY>(a.sv, st, ws, a.tmin, kInf);
+            int rc = wide_step<kLdsStack, ANY>(a.sv, st, ws, a.tmin, a.tmax);
             if (rc != kStepContinue) {
                 HitOut h;
                 if (rc == kStepDone) rc = wide_finish<ANY>(a.sv, st, a.tmin, h);
@@ -90,13 +95,14 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
 
 // Exact engine over a ray list (fallbacks) or over all rays (list == nullptr).
 __global__ __launch_bounds__(64) void k_trace_exact(TraceArgs a, const uint32_t* list, const uint32_t* count) {
-    uint32_t n = list ? *count : a.n;
+    uint32_t n = list ? *count : (a.count ? *a.count : a.n);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        uint32_t i = list ? list[j] : j;
+        uint32_t i = list ? list[j] : (a.queue ? a.queue[j] : j);
         vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
         vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
         HitOut h;
         trace_exact(a.sv, o, d, a.tmin, h);
+        if (a.any && h.prim != 0xFFFFFFFFu && !(h.t < a.tmax)) h.prim = 0xFFFFFFFFu;
         write_out(a, i, h);
     }
 }
@@ -116,9 +122,9 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
     __shared__ u32x2 stk[kCoopStack];
     const int lane = threadIdx.x;
     const SceneView& sv = a.sv;
-    uint32_t n = list ? *count : a.n;
+    uint32_t n = list ? *count : (a.count ? *a.count : a.n);
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
-        uint32_t ray = list ? list[j] : j;
+        uint32_t ray = list ? list[j] : (a.queue ? a.queue[j] : j);
         vf3 o = mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]);
         vf3 d = mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]);
         float bt = kInf, bu = 0.f, bv = 0.f;
@@ -236,6 +242,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                 bprim = oprim;
             }
         }
+        if (a.any && bprim != 0xFFFFFFFFu && !(bt < a.tmax)) bprim = 0xFFFFFFFFu;
         if (lane == 0) {
             HitOut h;
             h.prim = bprim;
@@ -245,6 +252,10 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
             write_out(a, ray, h);
         }
     }
+}
+
+__global__ void k_add_u64(uint64_t* acc, const uint32_t* v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *v;
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -294,6 +305,40 @@ int ensure_trace_scratch(vr_scene* s, uint32_t n) {
     return 0;
 }
 
+int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
+    if (j.n_max == 0) return 0;
+    int rc = ensure_trace_scratch(s, j.n_max);
+    if (rc) return rc;
+    TraceArgs a{s->view, j.o, j.d, j.n_max, j.tmin, j.any ? 1 : 0, j.prim, j.t, j.u, j.v, j.hit,
+                s->fallback_list, s->counters, s->counters + 1, s->spill, s->spill_lanes, j.queue, j.count, j.tmax};
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, s->counters, 4u);
+    uint32_t* serial_list = s->fallback_list + s->fallback_cap / 2;
+    if (j.engine == VR_ENGINE_WIDE) {
+        uint32_t blocks = 0;
+        rc = persist_grid(s, j.any, blocks);
+        if (rc) return rc;
+        blocks = std::min<uint32_t>(blocks, (j.n_max + kWave - 1) / kWave);
+        if (j.any)
+            hipLaunchKernelGGL(k_trace_persist<true>, dim3(blocks), dim3(kWave), 0, stream, a);
+        else
+            hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
+        // fallbacks: one wave per ray, grid reads the device-side count (no host sync);
+        // rays whose frontier outgrows LDS go to the serial exact kernel
+        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)s->fallback_list,
+                           (const uint32_t*)s->counters, serial_list, s->counters + 2);
+    } else {
+        uint32_t blocks = std::min<uint32_t>(j.n_max, 65535u);
+        hipLaunchKernelGGL(k_trace_exact_wave, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr,
+                           (const uint32_t*)nullptr, serial_list, s->counters + 2);
+    }
+    hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a, (const uint32_t*)serial_list,
+                       (const uint32_t*)(s->counters + 2));
+    if (j.fallback_accum && j.engine == VR_ENGINE_WIDE)
+        hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, stream, j.fallback_accum, (const uint32_t*)s->counters);
+    VR_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d, uint32_t* d_prim,
                  float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream) {
     uint32_t n = desc->num_rays;
@@ -302,34 +347,21 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
     if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE)
         return fail(VR_ERR_INVALID, "bad engine");
-    int rc = ensure_trace_scratch(s, n);
+    TraceJob j;
+    j.o = d_o;
+    j.d = d_d;
+    j.n_max = n;
+    j.tmin = desc->t_min;
+    j.tmax = kInf;
+    j.any = desc->mode == VR_TRACE_ANY;
+    j.engine = desc->engine;
+    j.prim = d_prim;
+    j.t = d_t;
+    j.u = d_u;
+    j.v = d_v;
+    j.hit = d_hit;
+    int rc = launch_trace(s, j, stream);
     if (rc) return rc;
-    TraceArgs a{s->view, d_o, d_d, n, desc->t_min, desc->mode == VR_TRACE_ANY ? 1 : 0, d_prim, d_t, d_u, d_v, d_hit,
-                s->fallback_list, s->counters, s->counters + 1, s->spill, s->spill_lanes};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, s->counters, 64u);
-    if (desc->engine == VR_ENGINE_WIDE) {
-        uint32_t blocks = 0;
-        rc = persist_grid(s, a.any != 0, blocks);
-        if (rc) return rc;
-        blocks = std::min<uint32_t>(blocks, (n + kWave - 1) / kWave);
-        if (a.any)
-            hipLaunchKernelGGL(k_trace_persist<true>, dim3(blocks), dim3(kWave), 0, stream, a);
-        else
-            hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
-        // fallbacks: one wave per ray, grid reads the device-side count (no host sync);
-        // rays whose frontier outgrows LDS go to the serial exact kernel
-        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)s->fallback_list,
-                           (const uint32_t*)s->counters, s->fallback_list + s->fallback_cap / 2, s->counters + 2);
-        hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a,
-                           (const uint32_t*)(s->fallback_list + s->fallback_cap / 2), (const uint32_t*)(s->counters + 2));
-    } else {
-        uint32_t blocks = std::min<uint32_t>(n, 65535u);
-        hipLaunchKernelGGL(k_trace_exact_wave, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr,
-                           (const uint32_t*)nullptr, s->fallback_list + s->fallback_cap / 2, s->counters + 2);
-        hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a,
-                           (const uint32_t*)(s->fallback_list + s->fallback_cap / 2), (const uint32_t*)(s->counters + 2));
-    }
-    VR_HIP_CHECK(hipGetLastError());
     VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     return 0;
 }

@@ -166,3 +166,95 @@ def microbench_rays(num_rays, res=2048, seed=3):
     o1, d1 = pinhole_rays(res, half)
     o2, d2 = incoherent_rays(num_rays - half, seed)
     return np.concatenate([o1, o2]), np.concatenate([d1, d2])
+
+
+def procedural_texture(size, seed, kind="albedo"):
+    """RGBA8 texture: smooth value noise (albedo) or a perturbed +Y normal map."""
+    rng = _rng(seed)
+    g = rng.random((9, 9, 3))
+    yy, xx = np.meshgrid(np.linspace(0, 8, size, endpoint=False), np.linspace(0, 8, size, endpoint=False), indexing="ij")
+    x0, y0 = xx.astype(int), yy.astype(int)
+    fx, fy = (xx - x0)[..., None], (yy - y0)[..., None]
+    v = (g[y0, x0] * (1 - fx) * (1 - fy) + g[y0, x0 + 1] * fx * (1 - fy) + g[y0 + 1, x0] * (1 - fx) * fy +
+         g[y0 + 1, x0 + 1] * fx * fy)
+    if kind == "normal":
+        n = np.stack([(v[..., 0] - 0.5) * 0.4, np.ones_like(v[..., 0]), (v[..., 1] - 0.5) * 0.4], -1)
+        n /= np.linalg.norm(n, axis=-1, keepdims=True)
+        v = n * 0.5 + 0.5
+    rgba = np.concatenate([v, np.ones_like(v[..., :1])], -1)
+    return (np.clip(rgba, 0, 1) * 255 + 0.5).astype(np.uint8)
+
+
+def procedural_sky(width, height, seed=6, sun_peak=1e4, sun_dir=(0.3, 0.8, 0.5), sun_sharpness=2000.0):
+    """RGB f32 equirect sky: log-normal-ish gradient plus a sun lobe, laid out
+    for HDRIEnvironment's lookup (row y at polar angle pi*(H-1-y)/H from -Y)."""
+    rng = _rng(seed)
+    y = np.arange(height)[:, None] + 0.5
+    x = np.arange(width)[None, :] + 0.5
+    th = np.pi * (height - 1 - y + 0.5) / height
+    ph = 2 * np.pi * x / width
+    d = np.stack([np.sin(th) * np.cos(ph - np.pi) * np.ones_like(ph), -np.cos(th) * np.ones_like(ph),
+                  -np.sin(th) * np.sin(ph - np.pi) * np.ones_like(ph)], -1)
+    up = np.clip(d[..., 1], -1, 1)
+    base = np.exp(rng.normal(0.0, 0.15, (height, width)))[..., None]
+    sky = (0.3 + 0.7 * np.clip(up, 0, 1))[..., None] * np.array([0.35, 0.55, 1.0]) * base
+    ground = (np.clip(-up, 0, 1) > 0)[..., None] * np.array([0.25, 0.22, 0.2]) * 0.5
+    s = np.asarray(sun_dir, np.float64)
+    s /= np.linalg.norm(s)
+    sun = sun_peak * np.exp(sun_sharpness * (np.sum(d * s, -1) - 1.0))[..., None] * np.array([1.0, 0.9, 0.7])
+    return np.ascontiguousarray((sky + ground + sun).astype(f32))
+
+
+def material_spheres(seed=9, env="hdri", dof=True, tex_size=64, sky=(64, 32)):
+    """Small scene exercising every material, textures, the HDRI and depth of field."""
+    rng = _rng(seed)
+    scene = Scene.empty()
+    alb = scene.add_image_texture(procedural_texture(tex_size, seed, "albedo"), SampleType.Bilinear)
+    nrm = scene.add_image_texture(procedural_texture(tex_size, seed + 1, "normal"), SampleType.Nearest)
+    g = 2.0
+    ground = Mesh(np.array([[-g, 0, -g], [-g, 0, g], [g, 0, g], [g, 0, -g]], f32), [[0, 1, 2], [2, 0, 3]],
+                  np.tile([0.0, 1.0, 0.0], (4, 1)).astype(f32),
+                  np.array([[0, 0], [0, 2.5], [2.5, 2.5], [2.5, 0]], f32))
+    s = scene.add_mesh(ground)
+    scene.add_object(scene.add_material(Materials.lambertian_texture(alb, nrm)), s)
+    mats = [Materials.metal((0.8, 0.6, 0.3), 0.3), Materials.dielectric(1.5), Materials.lambertian((0.7, 0.2, 0.2)),
+            Materials.lambertian_texture_no_normal(alb), Materials.metal((0.9, 0.9, 0.9), 0.0),
+            Materials.lambertian_bsdf((0.5, 0.8, 0.5))]
+    for i, m in enumerate(mats):
+        c = (-1.25 + 0.5 * i, 0.25 + 0.05 * rng.random(), 0.2 * rng.standard_normal())
+        s = scene.add_mesh(uv_sphere(c, 0.22, 24, 12))
+        scene.add_object(scene.add_material(m), s)
+    light = Mesh(np.array([[-0.3, 1.5, -0.3], [0.3, 1.5, -0.3], [0.3, 1.5, 0.3], [-0.3, 1.5, 0.3]], f32),
+                 [[0, 1, 2], [2, 3, 0]])
+    s = scene.add_mesh(light)
+    scene.add_object(scene.add_material(Materials.emissive(4.0)), s)
+    if env == "hdri":
+        scene.environment = Environments.hdri(procedural_sky(*sky, sun_peak=50.0, sun_sharpness=40.0))
+    elif env == "uniform":
+        scene.environment = Environments.uniform((0.4, 0.5, 0.7))
+    scene.camera = Camera.look_at((0.0, 1.0, 3.5), (0.0, 0.2, 0.0), (0.0, 1.0, 0.0), degrees_to_radians(45.0))
+    if dof:
+        scene.camera.dof = (0.03, (0.0, 0.3, 0.0))
+    return scene
+
+
+def cornell_zero_normal_quads(seed=1):
+    """The cornell room built with Surfaces::quad (Vertex::position: zero normals)."""
+    from .scene import Surfaces
+    s = 1.0 / 555.0
+    q = lambda *pts: Surfaces.quad(*[np.asarray(p, np.float64) * s for p in pts])
+    meshes = [q((0, 0, 0), (0, 0, 555), (555, 0, 555), (555, 0, 0)),
+              q((0, 0, 0), (0, 555, 0), (0, 555, 555), (0, 0, 555)),
+              q((555, 0, 0), (555, 0, 555), (555, 555, 555), (555, 555, 0)),
+              q((0, 0, 555), (0, 555, 555), (555, 555, 555), (555, 0, 555)),
+              q((0, 555, 0), (555, 555, 0), (555, 555, 555), (0, 555, 555)),
+              q((213, 554, 227), (213, 554, 332), (343, 554, 332), (343, 554, 227))]
+    rng = _rng(seed)
+    scene = Scene.empty()
+    for k, m in enumerate(meshes):
+        surf = scene.add_mesh(m)
+        mat = Materials.emissive(15.0) if k == len(meshes) - 1 else Materials.lambertian(rng.uniform(0.2, 0.8, 3))
+        scene.add_object(scene.add_material(mat), surf)
+    scene.camera = Camera.look_at((278 * s, 278 * s, -800 * s), (278 * s, 278 * s, 0.0), (0.0, 1.0, 0.0),
+                                  degrees_to_radians(40.0))
+    return scene
