@@ -58,6 +58,8 @@ def lib():
         L.orc_render_f64.argtypes = [_P, C.POINTER(N.vr_render_desc), _P, C.c_int, C.c_uint32, C.c_int, _P]
         L.orc_tonemap.argtypes = [C.POINTER(N.vr_post_desc), _P, _P]
         L.orc_num_threads.restype = C.c_int
+        L.orc_tex_sample_f32.argtypes = [_P, C.c_float, C.c_float, _P]
+        L.orc_env_sample_f32.argtypes = [C.c_uint32, C.c_uint32, _P, _P, _P]
         _lib = L
     return _lib
 

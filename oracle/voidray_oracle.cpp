@@ -964,3 +964,26 @@ void orc_tonemap(const vr_post_desc* p, const float* src, float* dst) {
 int orc_num_threads(void) { return hw_threads(0); }
 
 }  // extern "C"
+
+extern "C" {
+// Direct lookups for known-answer tests of texture.rs:82-98 and environments.rs:80-90.
+void orc_tex_sample_f32(const vr_texture* t, float u, float v, float* out) {
+    Texture<float> tex;
+    make_texture(tex, *t);
+    V3<float> c = tex_sample(tex, u, v);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+void orc_env_sample_f32(uint32_t w, uint32_t h, const float* data, const float* dir, float* out) {
+    Scene<float> s;
+    s.env_kind = VR_ENV_HDRI;
+    vr_texture t{w, h, VR_TEX_RGB32F, VR_SAMPLE_BILINEAR, data};
+    make_texture(s.env, t);
+    Ray<float> r{v3<float>(0, 0, 0), v3<float>(dir[0], dir[1], dir[2])};
+    V3<float> c = env_sample(s, r);
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+}
