@@ -516,11 +516,14 @@ template <class R> struct Scatter {
 // Material::scatter implementations (voidray_common/src/simple.rs, core/traits.rs:19-41)
 template <class R>
 Scatter<R> scatter(const Scene<R>& s, const Material<R>& m, const Ray<R>& ray, const Hit<R>& hit, PathRng& rng,
-                   RenderCounters& rc) {
+                   RenderCounters& rc, bool cosine_lobe = false, V3<R>* lobe = nullptr) {
     Scatter<R> out;
     switch (m.kind) {
         case VR_MAT_LAMBERTIAN: {
             V3<R> n = m.normal_tex >= 0 ? tex_sample(s.textures[m.normal_tex], hit.uv.x, hit.uv.y) : hit.normal;
+            // extended estimator: scatter about the normalised shading normal (pdf = cos/pi)
+            if (cosine_lobe) n = normalize(n);
+            if (lobe) *lobe = n;
             V3<R> dir = n + unit_sphere<R>(rng, rc.capped);
             if (near_zero(dir)) dir = n;
             out.ray = ray_new(hit.point, dir);
@@ -631,12 +634,186 @@ template <class R> Ray<R> cast_ray(const Scene<R>& s, R x, R y, PathRng& rng, Re
     return ray_new(origin, normalize(nd));
 }
 
+// ---- extended estimator (no counterpart in the reference) ---------------------
+// Environment importance sampling over the continuous lookup coordinates of
+// HDRIEnvironment::sample: cell (i, r) covers x in [i, i+1), y in [r-1, r), its
+// weight is luminance(lookup at the cell centre) * sin(theta_centre) + a floor
+// of 1e-3 * mean; theta = pi (H - 1 - y) / H, phi = 2 pi x / W, so
+// pdf_omega = pdf_cell * W * H / (2 pi^2 sin theta).  Prefix sums in FP64.
+template <class R> struct EnvTables {
+    std::vector<R> marg, cond, pdf;  // H+1, H*(W+1), H*W
+    uint32_t W = 0, H = 0;
+};
+
+template <class R> EnvTables<R> make_env_tables(const Texture<R>& env) {
+    EnvTables<R> T;
+    uint32_t W = env.w, H = env.h;
+    T.W = W;
+    T.H = H;
+    size_t len = (size_t)W * H;
+    auto lum = [&](size_t i) {
+        const V3<R>& c = env.img[i % len];
+        return 0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z;
+    };
+    std::vector<double> w(len);
+    double mean = 0;
+    for (uint32_t r = 0; r < H; ++r) {
+        double yc = (double)r - 0.5;
+        double th = M_PI * ((double)H - 1.0 - yc) / (double)H;
+        double st = std::sin(th);
+        for (uint32_t i = 0; i < W; ++i) {
+            double x = i + 0.5, y = yc;
+            size_t x0 = std::min<size_t>(x >= 0 ? (size_t)x : 0, W - 1);
+            size_t y0 = std::min<size_t>(y >= 0 ? (size_t)y : 0, H - 1);
+            double ax = x - (double)x0, ay = y - (double)y0;
+            double a = lum(y0 * W + x0) * (1 - ax) + lum(y0 * W + x0 + 1) * ax;
+            double b = lum((y0 + 1) * W + x0) * (1 - ax) + lum((y0 + 1) * W + x0 + 1) * ax;
+            double l = std::max(0.0, a * (1 - ay) + b * ay);
+            if (!std::isfinite(l)) l = 0;
+            w[(size_t)r * W + i] = l * st;
+            mean += l * st;
+        }
+    }
+    mean /= (double)len;
+    double fl = std::max(mean * 1e-3, 1e-12);
+    for (auto& x : w) x += fl;
+    std::vector<double> m(H + 1, 0.0), c((size_t)H * (W + 1), 0.0);
+    for (uint32_t r = 0; r < H; ++r) {
+        double* cr = &c[(size_t)r * (W + 1)];
+        for (uint32_t i = 0; i < W; ++i) cr[i + 1] = cr[i] + w[(size_t)r * W + i];
+        m[r + 1] = m[r] + cr[W];
+    }
+    double total = m[H];
+    T.marg.resize(H + 1);
+    T.cond.resize((size_t)H * (W + 1));
+    T.pdf.resize(len);
+    for (uint32_t r = 0; r <= H; ++r) T.marg[r] = R(m[r] / total);
+    T.marg[H] = R(1);
+    for (uint32_t r = 0; r < H; ++r) {
+        for (uint32_t i = 0; i <= W; ++i) T.cond[(size_t)r * (W + 1) + i] = R(c[(size_t)r * (W + 1) + i] / c[(size_t)r * (W + 1) + W]);
+        T.cond[(size_t)r * (W + 1) + W] = R(1);
+        for (uint32_t i = 0; i < W; ++i) T.pdf[(size_t)r * W + i] = R(w[(size_t)r * W + i] / total);
+    }
+    return T;
+}
+
+template <class R> uint32_t cdf_find(const R* cdf, uint32_t n, R u) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class R> R env_pdf_omega(const EnvTables<R>& T, R x, R y, R sin_t) {
+    if (!(sin_t > R(0))) return R(0);
+    int64_t i = (int64_t)std::floor(x), r = (int64_t)std::floor(y) + 1;
+    i = std::min<int64_t>(std::max<int64_t>(i, 0), T.W - 1);
+    r = std::min<int64_t>(std::max<int64_t>(r, 0), T.H - 1);
+    R pc = T.pdf[(size_t)r * T.W + (size_t)i];
+    return pc * (R(T.W) * R(T.H)) / (R(2) * pi_v<R>() * pi_v<R>() * sin_t);
+}
+
+template <class R> R env_pdf_dir(const EnvTables<R>& T, V3<R> dir) {
+    V3<R> d = normalize(dir);
+    R th = std::acos(-d.y);
+    R ph = std::atan2(-d.z, d.x) + pi_v<R>();
+    R x = (ph / (R(2) * pi_v<R>())) * R(T.W);
+    R y = R(T.H - 1) - (th / pi_v<R>()) * R(T.H);
+    return env_pdf_omega(T, x, y, std::sin(th));
+}
+
+template <class R> V3<R> env_sample_dir(const EnvTables<R>& T, R u1, R u2, R& pdf) {
+    uint32_t r = cdf_find(T.marg.data(), T.H, u1);
+    R m0 = T.marg[r], m1 = T.marg[r + 1];
+    const R* c = &T.cond[(size_t)r * (T.W + 1)];
+    uint32_t i = cdf_find(c, T.W, u2);
+    R c0 = c[i], c1 = c[i + 1];
+    R fx = c1 > c0 ? (u2 - c0) / (c1 - c0) : R(0.5);
+    R fy = m1 > m0 ? (u1 - m0) / (m1 - m0) : R(0.5);
+    R top = R(0.99999994f);
+    fx = std::min(std::max(fx, R(0)), top);
+    fy = std::min(std::max(fy, R(0)), top);
+    R x = R(i) + fx, y = (R(r) - R(1)) + fy;
+    R th = pi_v<R>() * (R(T.H) - R(1) - y) / R(T.H);
+    R ph = R(2) * pi_v<R>() * x / R(T.W);
+    R st = std::sin(th), ct = std::cos(th);
+    R sp = std::sin(ph - pi_v<R>()), cp = std::cos(ph - pi_v<R>());
+    pdf = env_pdf_omega(T, x, y, st);
+    return v3(st * cp, -ct, -st * sp);
+}
+
+// Extended estimator: forward throughput, environment NEE with MIS (power
+// heuristic) at Lambertian vertices, Russian roulette; firefly clamp off.
+// Same order of random draws per bounce as the GPU: [NEE: 2] [material] [RR: 1].
+template <class R>
+V3<R> trace_ext(const Scene<R>& s, const EnvTables<R>* T, const vr_render_desc& d, Ray<R> ray, PathRng& rng,
+                RenderCounters& rc, bool prune) {
+    V3<R> L = v3<R>(0, 0, 0), A = v3<R>(1, 1, 1);
+    R pdf_prev = R(0);
+    bool env_is = (d.flags & VR_RENDER_ENV_IS) && T && s.env_kind == VR_ENV_HDRI;
+    for (uint32_t depth = 0; depth < d.max_bounces; ++depth) {
+        Hit<R> hit;
+        ++rc.segments;
+        if (!scene_hit(s, ray, R(0.00001f), hit, prune, rc.q)) {
+            ++rc.misses;
+            if (s.env_kind == VR_ENV_NONE) break;
+            V3<R> env = env_sample(s, ray);
+            R w = R(1);
+            if (env_is && pdf_prev > R(0)) {
+                R pe = env_pdf_dir(*T, ray.d);
+                w = (pdf_prev * pdf_prev) / (pdf_prev * pdf_prev + pe * pe);
+            }
+            L = L + mulv(A, env) * w;
+            break;
+        }
+        const Material<R>& m = s.materials[s.objects[hit.surface].material];
+        if (m.kind == VR_MAT_EMISSION) {
+            L = L + mulv(A, m.color);
+            break;
+        }
+        if (m.kind == VR_MAT_LAMBERTIAN && env_is) {
+            R u1 = R(rng.uniform()), u2 = R(rng.uniform());
+            V3<R> ns = m.normal_tex >= 0 ? tex_sample(s.textures[m.normal_tex], hit.uv.x, hit.uv.y) : hit.normal;
+            ns = normalize(ns);
+            R pe = 0;
+            V3<R> wi = env_sample_dir(*T, u1, u2, pe);
+            R cw = dot(ns, wi);
+            if (cw > R(0) && pe > R(0)) {
+                R pb = cw / pi_v<R>();
+                R w = (pe * pe) / (pe * pe + pb * pb);
+                V3<R> albedo = m.albedo_tex >= 0 ? tex_sample(s.textures[m.albedo_tex], hit.uv.x, hit.uv.y) : m.color;
+                V3<R> contrib = mulv(mulv(A, albedo), env_sample(s, Ray<R>{hit.point, wi})) * (pb * w / pe);
+                Hit<R> sh;
+                ++rc.segments;
+                if (!scene_hit(s, Ray<R>{hit.point, wi}, R(0.00001f), sh, true, rc.q)) L = L + contrib;
+            }
+        }
+        V3<R> lobe = v3<R>(0, 0, 0);
+        Scatter<R> sc = scatter(s, m, ray, hit, rng, rc, true, &lobe);
+        if (!sc.has_ray || depth + 1 >= d.max_bounces) break;
+        A = mulv(A, sc.att);
+        R pb = m.kind == VR_MAT_LAMBERTIAN ? std::max(R(0), dot(lobe, sc.ray.d)) / pi_v<R>() : R(0);
+        if ((d.flags & VR_RENDER_RUSSIAN_ROULETTE) && depth >= d.rr_depth) {
+            R q = std::min(R(0.95f), std::max(A.x, std::max(A.y, A.z)));
+            if (R(rng.uniform()) >= q) break;
+            A = A / q;
+        }
+        pdf_prev = pb;
+        ray = sc.ray;
+    }
+    return L;
+}
+
 template <class R> struct RenderJob {
     const Scene<R>* s;
     const vr_render_desc* d;
     R* rgba;
     uint32_t row_step;
     bool prune;
+    const EnvTables<R>* env_tables;
 };
 
 // iterative_render per-pixel body (render/iterative.rs:25-51)
@@ -651,12 +828,14 @@ template <class R> void render_pixel(const RenderJob<R>& job, uint32_t index, Re
     RenderParams p{d.max_bounces, d.firefly_clamp, d.render_mode};
     V3<R> color = v3<R>(0, 0, 0);
     R lo = R(-1) / dd, hi = R(1) / dd;
+    bool ext = (d.flags & (VR_RENDER_ENV_IS | VR_RENDER_RUSSIAN_ROULETTE)) != 0;
     for (uint32_t k = 0; k < d.samples; ++k) {
         PathRng rng(d.seed, d.request, index, d.sample_offset + k);
         R dx = R(rng.uniform()) * (hi - lo) + lo;
         R dy = R(rng.uniform()) * (hi - lo) + lo;
         Ray<R> r = cast_ray(*job.s, xf + dx, yf + dy, rng, rc);
-        color = color + trace_rec(*job.s, p, r, 0, rng, rc, job.prune);
+        if (ext) color = color + trace_ext(*job.s, job.env_tables, d, r, rng, rc, job.prune);
+        else color = color + trace_rec(*job.s, p, r, 0, rng, rc, job.prune);
     }
     color = color * (R(1) / R(d.total_samples));
     R* px = job.rgba + 4 * (size_t)index;
@@ -871,7 +1050,10 @@ void orc_trace_bruteforce_f32(void* sp, uint32_t n, const float* o, const float*
 // rows y % row_step == 0 (bounded CPU-baseline samples).  stats (optional):
 // segments, misses, capped, boxes, tris.
 double orc_render_f32(void* s, const vr_render_desc* d, float* rgba, int nthreads, uint32_t row_step, uint64_t* stats) {
-    RenderJob<float> job{(Scene<float>*)s, d, rgba, row_step, false};
+    auto* sc = (Scene<float>*)s;
+    std::unique_ptr<EnvTables<float>> T;
+    if (sc->env_kind == VR_ENV_HDRI) T.reset(new EnvTables<float>(make_env_tables(sc->env)));
+    RenderJob<float> job{sc, d, rgba, row_step, false, T.get()};
     RenderCounters rc;
     auto t0 = std::chrono::steady_clock::now();
     render_all(job, hw_threads(nthreads), rc);
@@ -888,7 +1070,10 @@ double orc_render_f32(void* s, const vr_render_desc* d, float* rgba, int nthread
 
 double orc_render_f64(void* s, const vr_render_desc* d, double* rgba, int nthreads, uint32_t row_step, int prune,
                       uint64_t* stats) {
-    RenderJob<double> job{(Scene<double>*)s, d, rgba, row_step, prune != 0};
+    auto* sc = (Scene<double>*)s;
+    std::unique_ptr<EnvTables<double>> T;
+    if (sc->env_kind == VR_ENV_HDRI) T.reset(new EnvTables<double>(make_env_tables(sc->env)));
+    RenderJob<double> job{sc, d, rgba, row_step, prune != 0, T.get()};
     RenderCounters rc;
     auto t0 = std::chrono::steady_clock::now();
     render_all(job, hw_threads(nthreads), rc);

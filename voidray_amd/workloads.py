@@ -33,6 +33,27 @@ def _normalize_rows(v):
     return (v * inv[:, None]).astype(f32)
 
 
+def _sphere_topology(slices, stacks):
+    W = slices + 1
+    a, b = np.meshgrid(np.arange(stacks), np.arange(slices), indexing="ij")
+    v00, v01, v10, v11 = a * W + b, a * W + b + 1, (a + 1) * W + b, (a + 1) * W + b + 1
+    t1 = np.stack([v00, v10, v01], -1)[1:].reshape(-1, 3)        # skipped on the top pole row
+    t2 = np.stack([v01, v10, v11], -1)[:-1].reshape(-1, 3)       # skipped on the bottom pole row
+    # interleave per cell in row order: (t1, t2) of cell (a, b), as a scanline mesh writer would
+    rows = []
+    for r in range(stacks):
+        if r == 0:
+            rows.append(t2[:slices])
+        elif r == stacks - 1:
+            rows.append(t1[(r - 1) * slices:r * slices])
+        else:
+            x = np.empty((2 * slices, 3), np.int64)
+            x[0::2] = t1[(r - 1) * slices:r * slices]
+            x[1::2] = t2[r * slices:(r + 1) * slices]
+            rows.append(x)
+    return np.concatenate(rows).astype(np.uint32)
+
+
 def uv_sphere(center, radius, slices=32, stacks=16):
     """UV sphere: (stacks+1) x (slices+1) vertices with seam duplicates; pole
     rows produce one triangle per cell (2*slices*(stacks-1) triangles)."""
@@ -44,16 +65,76 @@ def uv_sphere(center, radius, slices=32, stacks=16):
     n = np.stack([np.sin(th) * np.cos(ph), np.cos(th) * np.ones_like(ph), np.sin(th) * np.sin(ph)], -1).reshape(-1, 3)
     pos = (c + radius * n).astype(f32)
     uv = np.stack([(j / slices) * np.ones_like(th), 1 - (i / stacks) * np.ones_like(ph)], -1).reshape(-1, 2)
-    tris = []
-    W = slices + 1
-    for a in range(stacks):
-        for b in range(slices):
-            v00, v01, v10, v11 = a * W + b, a * W + b + 1, (a + 1) * W + b, (a + 1) * W + b + 1
-            if a != 0:
-                tris.append((v00, v10, v01))
-            if a != stacks - 1:
-                tris.append((v01, v10, v11))
-    return Mesh(pos, np.asarray(tris, np.uint32), n.astype(f32), uv.astype(f32))
+    return Mesh(pos, _sphere_topology(slices, stacks), n.astype(f32), uv.astype(f32))
+
+
+def fbm_heightfield(n, octaves=6, seed=4, amplitude=0.25):
+    """(n+1) x (n+1) heights over [-1, 1]^2: value-noise fBm, octave o on a
+    (2^(o+2)+1)^2 lattice with amplitude * 0.5^o, bilinear interpolation."""
+    rng = _rng(seed)
+    u = np.linspace(0.0, 1.0, n + 1)
+    h = np.zeros((n + 1, n + 1))
+    for o in range(octaves):
+        m = 2 ** (o + 2)
+        g = rng.random((m + 1, m + 1)) * 2 - 1
+        x = u * m
+        x0 = np.minimum(x.astype(int), m - 1)
+        fx = x - x0
+        fx = fx * fx * (3 - 2 * fx)
+        a = g[x0][:, x0] * (1 - fx)[None, :] + g[x0][:, x0 + 1] * fx[None, :]
+        b = g[x0 + 1][:, x0] * (1 - fx)[None, :] + g[x0 + 1][:, x0 + 1] * fx[None, :]
+        h += amplitude * 0.5 ** o * (a * (1 - fx)[:, None] + b * fx[:, None])
+    return h
+
+
+def heightfield_mesh(h, uv_repeat=4.0):
+    n = h.shape[0] - 1
+    x = np.linspace(-1.0, 1.0, n + 1)
+    X, Z = np.meshgrid(x, x, indexing="xy")  # row = z, col = x
+    pos = np.stack([X, h, Z], -1).reshape(-1, 3)
+    dx = 2.0 / n
+    gz, gx = np.gradient(h, dx, dx)
+    nrm = np.stack([-gx, np.ones_like(h), -gz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    uv = np.stack([(X + 1) / 2 * uv_repeat, (Z + 1) / 2 * uv_repeat], -1).reshape(-1, 2)
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v00 = r * (n + 1) + c
+    v01, v10, v11 = v00 + 1, v00 + n + 1, v00 + n + 2
+    tris = np.stack([np.stack([v00, v10, v11], -1), np.stack([v00, v11, v01], -1)], 2).reshape(-1, 3)
+    return Mesh(pos.astype(f32), tris.astype(np.uint32), nrm.reshape(-1, 3).astype(f32), uv.astype(f32))
+
+
+def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048, env_size=(2048, 1024),
+                    sun_peak=1e4, seed=4):
+    """fBm-displaced heightfield (2*grid^2 triangles), n_spheres UV spheres with
+    materials 1/3 Lambertian, 1/3 metal (fuzz U(0,1)), 1/3 dielectric 1.5; the
+    terrain is Lambertian with procedural RGBA8 albedo and normal textures; an
+    RGB f32 sky with a sun lobe of peak `sun_peak`."""
+    rng = _rng(seed + 1)
+    h = fbm_heightfield(grid, seed=seed)
+    scene = Scene.empty()
+    alb = scene.add_image_texture(procedural_texture(tex_size, seed + 2, "albedo"), SampleType.Bilinear)
+    nrm = scene.add_image_texture(procedural_texture(tex_size, seed + 3, "normal"), SampleType.Bilinear)
+    s = scene.add_mesh(heightfield_mesh(h))
+    scene.add_object(scene.add_material(Materials.lambertian_texture(alb, nrm)), s)
+    n = h.shape[0] - 1
+    for k in range(n_spheres):
+        x, z = rng.uniform(-0.85, 0.85, 2)
+        r = rng.uniform(0.02, 0.06)
+        ix, iz = int((x + 1) / 2 * n), int((z + 1) / 2 * n)
+        y = h[iz, ix] + 0.8 * r
+        kind = k % 3
+        if kind == 0:
+            mat = Materials.lambertian(rng.uniform(0.2, 0.8, 3))
+        elif kind == 1:
+            mat = Materials.metal(rng.uniform(0.5, 0.95, 3), rng.uniform(0.0, 1.0))
+        else:
+            mat = Materials.dielectric(1.5)
+        s = scene.add_mesh(uv_sphere((x, y, z), r, *sphere_res))
+        scene.add_object(scene.add_material(mat), s)
+    scene.environment = Environments.hdri(procedural_sky(*env_size, seed=seed + 2, sun_peak=sun_peak))
+    scene.camera = Camera.look_at((0.0, 0.9, 2.3), (0.0, 0.0, 0.1), (0.0, 1.0, 0.0), degrees_to_radians(50.0))
+    return scene
 
 
 def _quad_mesh(p0, p1, p2, p3):
