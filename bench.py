@@ -57,6 +57,57 @@ def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
                       f"1,048,576-triangle scene; reference median-split tree built in {build_s:.1f} s (untimed)"}
 
 
+def _render_legs(ctx, cpu=True, c3_spp=64):
+    """Path-integration legs reported beside the headline metric:
+    * cornell 512x512, 16 spp, depth 8, reference estimator (firefly clamp 3);
+    * terrain+spheres 1920x1080, 64 spp, depth 16, RR from 4, env IS + MIS.
+    Device time of vr_render (HIP events), samples/s and segments (rays)/s."""
+    import numpy as np
+
+    from voidray_amd import _native as N
+    from voidray_amd import workloads as W
+    from voidray_amd.device import make_render_desc
+    from voidray_amd.scene import RenderSettings
+    out = {}
+    c1 = W.cornell_room()
+    a1 = c1.build_acceleration(ctx)
+    s1 = RenderSettings(total_samples=16, max_bounces=8)
+    img = np.zeros((512, 512, 4), np.float32)
+    a1.render(img, make_render_desc(512, 512, 1, 16, s1, seed=99))  # warm-up
+    img[:] = 0
+    st = a1.render(img, make_render_desc(512, 512, 16, 16, s1, seed=1))
+    out["cornell_512_16spp"] = {"ms": round(st["kernel_ms"], 3),
+                                "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
+                                "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2),
+                                "segments": st["segments"], "exact_engine_rays": st["fallback_rays"]}
+    t0 = time.time()
+    c3 = W.terrain_spheres()
+    a3 = c3.build_acceleration(ctx)
+    build3 = time.time() - t0
+    s3 = RenderSettings(total_samples=c3_spp, max_bounces=16, firefly_clamp=float("inf"))
+    ext = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+    img3 = np.zeros((1080, 1920, 4), np.float32)
+    a3.render(img3, make_render_desc(1920, 1080, 1, c3_spp, s3, seed=99, flags=ext, rr_depth=4))  # warm-up
+    img3[:] = 0
+    st = a3.render(img3, make_render_desc(1920, 1080, c3_spp, c3_spp, s3, seed=1, flags=ext, rr_depth=4))
+    leg = {"ms": round(st["kernel_ms"], 2), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
+           "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
+           "exact_engine_rays": st["fallback_rays"], "triangles": c3.num_triangles, "scene_build_s": round(build3, 2)}
+    if cpu:
+        import oracle
+        threads = min(16, oracle.num_threads())
+        orc = oracle.OracleScene(c3)
+        # bounded sample: 1 spp on every 3rd row (360 rows x 1920 pixels), same estimator, reference traversal
+        d1 = make_render_desc(1920, 1080, 1, c3_spp, s3, seed=5, flags=ext, rr_depth=4)
+        _, sec, ost = orc.render(d1, threads=threads, row_step=3)
+        n = 1920 * 360
+        leg["cpu_baseline"] = {"samples_per_s": round(n / sec, 1), "cores": threads, "kind": "port",
+                               "sample": f"{n} samples (1 spp, every 3rd row), reference median-split traversal"}
+        leg["gpu_over_cpu"] = round(leg["samples_per_s"] / (n / sec), 1)
+    out["terrain_1080p_64spp"] = leg
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +116,7 @@ def main():
     ap.add_argument("--tris", type=int, default=1 << 20)
     ap.add_argument("--rays", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-integration legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -132,6 +184,7 @@ def main():
     mrays = n / (ms * 1e-3) / 1e6
     achieved = BYTES_PER_RAY_CLOSEST * n / (ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
+    renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),
@@ -162,6 +215,7 @@ def main():
             "scene_stats": acc.stats(),
             "wall_s": round(wall, 3),
             "gpu_over_cpu": (round(mrays / cpu["value"], 1) if cpu else None),
+            "render": renders,
         },
     }
     print(json.dumps(out), flush=True)

@@ -87,12 +87,13 @@ def fbm_heightfield(n, octaves=6, seed=4, amplitude=0.25):
     return h
 
 
-def heightfield_mesh(h, uv_repeat=4.0):
+def heightfield_mesh(h, uv_repeat=4.0, half=1.0):
     n = h.shape[0] - 1
-    x = np.linspace(-1.0, 1.0, n + 1)
+    x = np.linspace(-half, half, n + 1)
     X, Z = np.meshgrid(x, x, indexing="xy")  # row = z, col = x
     pos = np.stack([X, h, Z], -1).reshape(-1, 3)
-    dx = 2.0 / n
+    dx = 2.0 * half / n
+    X, Z = X / half, Z / half
     gz, gx = np.gradient(h, dx, dx)
     nrm = np.stack([-gx, np.ones_like(h), -gz], -1)
     nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
@@ -105,23 +106,28 @@ def heightfield_mesh(h, uv_repeat=4.0):
 
 
 def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048, env_size=(2048, 1024),
-                    sun_peak=1e4, seed=4):
+                    sun_peak=1e4, seed=4, scale=10.0):
     """fBm-displaced heightfield (2*grid^2 triangles), n_spheres UV spheres with
     materials 1/3 Lambertian, 1/3 metal (fuzz U(0,1)), 1/3 dielectric 1.5; the
     terrain is Lambertian with procedural RGBA8 albedo and normal textures; an
-    RGB f32 sky with a sun lobe of peak `sun_peak`."""
+    RGB f32 sky with a sun lobe of peak `sun_peak`.
+
+    The scene spans [-scale, scale]^2.  At unit scale the 1024-grid triangles
+    (edges ~2e-3) would all be invisible: Triangle::hit uses t_min = 1e-5 as
+    its determinant epsilon (mesh.rs:156), and |e1 . (d x e2)| <= |e1||e2|."""
     rng = _rng(seed + 1)
-    h = fbm_heightfield(grid, seed=seed)
+    h = fbm_heightfield(grid, seed=seed) * scale
     scene = Scene.empty()
     alb = scene.add_image_texture(procedural_texture(tex_size, seed + 2, "albedo"), SampleType.Bilinear)
     nrm = scene.add_image_texture(procedural_texture(tex_size, seed + 3, "normal"), SampleType.Bilinear)
-    s = scene.add_mesh(heightfield_mesh(h))
+    s = scene.add_mesh(heightfield_mesh(h, half=scale))
     scene.add_object(scene.add_material(Materials.lambertian_texture(alb, nrm)), s)
     n = h.shape[0] - 1
     for k in range(n_spheres):
         x, z = rng.uniform(-0.85, 0.85, 2)
         r = rng.uniform(0.02, 0.06)
         ix, iz = int((x + 1) / 2 * n), int((z + 1) / 2 * n)
+        x, z, r = x * scale, z * scale, r * scale
         y = h[iz, ix] + 0.8 * r
         kind = k % 3
         if kind == 0:
@@ -133,7 +139,8 @@ def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048
         s = scene.add_mesh(uv_sphere((x, y, z), r, *sphere_res))
         scene.add_object(scene.add_material(mat), s)
     scene.environment = Environments.hdri(procedural_sky(*env_size, seed=seed + 2, sun_peak=sun_peak))
-    scene.camera = Camera.look_at((0.0, 0.9, 2.3), (0.0, 0.0, 0.1), (0.0, 1.0, 0.0), degrees_to_radians(50.0))
+    scene.camera = Camera.look_at((0.0, 0.9 * scale, 2.3 * scale), (0.0, 0.0, 0.1 * scale), (0.0, 1.0, 0.0),
+                                  degrees_to_radians(50.0))
     return scene
 
 
