@@ -117,6 +117,9 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-integration legs")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter runs: 3 closest-hit queries of the workload, then a 1 GiB device copy "
+                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -173,8 +176,26 @@ def main():
         wall = time.perf_counter() - t1
         return ev0.elapsed_time(ev1) / steps, wall
 
+    if args.pmc_probe:
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": "done", "copy_bytes": 4 << 28, "rays": n}), flush=True)
+        return
+
     ms, wall = timed(step, args.steps, args.warmup)
     fallbacks = acc.last_fallbacks()
+    # duration of the dominant kernel alone (HIP events the library records around it)
+    kern = []
+    for _ in range(args.steps):
+        step()
+        kern.append(acc.last_wide_ms())
+    kernel_ms = sum(kern) / len(kern)
     half = n // 2
     ms_coh, _ = timed(lambda: step(0, half), max(5, args.steps // 2), 1)
     ms_inc, _ = timed(lambda: step(half, n - half), max(5, args.steps // 2), 1)
@@ -182,7 +203,7 @@ def main():
     hits = int((prim != -1).sum().item())
 
     mrays = n / (ms * 1e-3) / 1e6
-    achieved = BYTES_PER_RAY_CLOSEST * n / (ms * 1e-3) / 1e9
+    achieved = BYTES_PER_RAY_CLOSEST * n / (kernel_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     out = {
@@ -202,7 +223,9 @@ def main():
                    "rays_mix": "1/2 coherent 2048^2 pinhole, 1/2 incoherent", "bvh": "SAH 4-wide, 8-bit quantised",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _traffic_from_profiles()},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _traffic_from_profiles(),
+                     "kernel": "k_trace_persist<false>", "kernel_ms": round(kernel_ms, 4),
+                     "bytes_per_ray": BYTES_PER_RAY_CLOSEST},
         "cpu_baseline": cpu,
         "extra": {
             "coherent_mrays_s": round(half / (ms_coh * 1e-3) / 1e6, 2),

@@ -186,8 +186,12 @@ int vr_trace_device_async(vr_scene* scene, const vr_trace_desc* desc,
                           const float* d_origins, const float* d_directions,
                           uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
                           uint8_t* d_hit, void* stream);
-/* Rays of the last trace that needed the exact fallback engine. */
+/* Rays of the last trace that needed the exact fallback engine (call after
+ * the stream of that trace has been synchronised). */
 int vr_trace_last_fallbacks(vr_scene* scene, uint32_t* count);
+/* Device time (ms, HIP events on the trace's stream) of the wide-engine kernel
+ * of the last vr_trace / vr_trace_device_async; waits for that kernel. */
+int vr_trace_last_timing(vr_scene* scene, float* wide_ms);
 
 /* ---- rendering (render/iterative.rs:11-55, core/tracer.rs:9-56) ------- */
 

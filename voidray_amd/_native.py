@@ -108,6 +108,7 @@ SIGNATURES = {
     "vr_trace": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P]),
     "vr_trace_device_async": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P, _P]),
     "vr_trace_last_fallbacks": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "vr_trace_last_timing": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "vr_render": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats)]),
     "vr_render_device_async": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats), _P]),
     "vr_render_last_stats": (C.c_int, [_P, C.POINTER(vr_render_stats)]),

@@ -162,6 +162,14 @@ int vr_trace_last_fallbacks(vr_scene* s, uint32_t* count) {
     return 0;
 }
 
+int vr_trace_last_timing(vr_scene* s, float* wide_ms) {
+    if (!s || !wide_ms) return fail(VR_ERR_INVALID, "null argument");
+    if (!s->trace_timed || !s->trace_ev[1]) return fail(VR_ERR_INVALID, "no timed wide-engine trace yet");
+    VR_HIP_CHECK(hipEventSynchronize(s->trace_ev[1]));
+    VR_HIP_CHECK(hipEventElapsedTime(wide_ms, s->trace_ev[0], s->trace_ev[1]));
+    return 0;
+}
+
 int vr_render_device_async(vr_scene* s, const vr_render_desc* desc, float* d_rgba, vr_render_stats* stats,
                            void* stream) {
     return guarded([&] {

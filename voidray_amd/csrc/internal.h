@@ -144,6 +144,8 @@ struct vr_scene {
     u32x2* spill = nullptr;             // per-lane spill stacks of the persistent traversal
     uint32_t spill_lanes = 0;
     uint32_t persist_blocks[2] = {0, 0};  // resident blocks of the persistent kernels
+    hipEvent_t trace_ev[2] = {nullptr, nullptr};  // around the wide kernel of the last vr_trace*
+    bool trace_timed = false;
     uint32_t last_fallbacks = 0;
     // host copies used by the render driver
     uint32_t num_prims = 0;
@@ -190,6 +192,7 @@ struct TraceJob {
     float *t = nullptr, *u = nullptr, *v = nullptr;
     uint8_t* hit = nullptr;
     uint64_t* fallback_accum = nullptr;  // optional: += number of exact-engine rays
+    bool timed = false;                  // record trace_ev[0..1] around the wide kernel
 };
 int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream);
 int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d, uint32_t* d_prim,

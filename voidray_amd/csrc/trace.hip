@@ -318,10 +318,12 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
         rc = persist_grid(s, j.any, blocks);
         if (rc) return rc;
         blocks = std::min<uint32_t>(blocks, (j.n_max + kWave - 1) / kWave);
+        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[0], stream));
         if (j.any)
             hipLaunchKernelGGL(k_trace_persist<true>, dim3(blocks), dim3(kWave), 0, stream, a);
         else
             hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
+        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[1], stream));
         // fallbacks: one wave per ray, grid reads the device-side count (no host sync);
         // rays whose frontier outgrows LDS go to the serial exact kernel
         hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)s->fallback_list,
@@ -360,6 +362,12 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     j.u = d_u;
     j.v = d_v;
     j.hit = d_hit;
+    if (!s->trace_ev[0]) {
+        VR_HIP_CHECK(hipEventCreate(&s->trace_ev[0]));
+        VR_HIP_CHECK(hipEventCreate(&s->trace_ev[1]));
+    }
+    j.timed = desc->engine == VR_ENGINE_WIDE;
+    s->trace_timed = j.timed;
     int rc = launch_trace(s, j, stream);
     if (rc) return rc;
     VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

@@ -87,6 +87,12 @@ class SceneAcceleration:
         N.check(N.lib().vr_trace_last_fallbacks(self.handle, C.byref(c)))
         return c.value
 
+    def last_wide_ms(self):
+        """Device time of the wide-engine kernel of the last trace (HIP events)."""
+        ms = C.c_float()
+        N.check(N.lib().vr_trace_last_timing(self.handle, C.byref(ms)))
+        return ms.value
+
     def trace_device(self, d_origins, d_directions, n, any_hit=False, t_min=1e-5, engine=N.VR_ENGINE_WIDE,
                      d_prim=None, d_t=None, d_u=None, d_v=None, d_hit=None, stream=None):
         """Device pointers (ints) in/out, enqueued on `stream` (None = context stream)."""
