@@ -202,6 +202,7 @@ int vr_trace_last_timing(vr_scene* scene, float* wide_ms);
 #define VR_RENDER_ENV_IS 2u         /* environment importance sampling + MIS     */
 #define VR_RENDER_RUSSIAN_ROULETTE 4u
 #define VR_RENDER_ALPHA_PER_CALL 8u /* alpha += 1 per call, as iterative.rs:51   */
+#define VR_RENDER_OVERWRITE 16u     /* store the frame instead of adding it (alpha = 1 with ALPHA_PER_CALL) */
 
 typedef struct vr_render_desc {
     uint32_t width, height;
@@ -235,9 +236,51 @@ int vr_render(vr_scene* scene, const vr_render_desc* desc, float* rgba_accum,
  * stream); `stats` is not filled (use vr_render_last_stats). */
 int vr_render_device_async(vr_scene* scene, const vr_render_desc* desc,
                            float* d_rgba_accum, vr_render_stats* stats, void* stream);
-/* Counters of the last render on this scene (waits for the context stream;
- * renders enqueued on other streams must be synchronised by the caller). */
+/* Counters of the last render or batch on the scene's context (waits for its
+ * statistics read-back). */
 int vr_render_last_stats(vr_scene* scene, vr_render_stats* stats);
+
+/* ---- batched serving ----------------------------------------------------
+ * The reference serves one frame at a time: RenderThread::one_shot
+ * (render/renderer.rs:67-107) calls iterative_render once per request.  These
+ * entry points serve many requests (different scenes, sizes, spp, seeds) in
+ * one launch: requests on the same scene share wavefront chunks, the whole
+ * batch is enqueued without host synchronisation and can be captured into a
+ * hipGraph and relaunched.  Each request's image equals its standalone
+ * vr_render_device_async result.  Render calls on one context share its
+ * scratch: enqueue them on one stream (or synchronise between streams). */
+
+typedef struct vr_request {
+    vr_scene* scene;
+    vr_render_desc desc;
+    float* d_rgba;            /* device target, width*height*4 f32 */
+} vr_request;
+
+typedef struct vr_tuning {
+    uint32_t chunk_paths;     /* paths per wavefront chunk (queue length); 0 = 2^22 */
+    uint32_t block;           /* workgroup size of raygen/shade/accumulate; 0 = 256 */
+} vr_tuning;
+
+#define VR_BATCH_GRAPH 1u          /* capture into a hipGraph and launch it (relaunchable) */
+#define VR_BATCH_TIMESTAMPS 2u     /* record per-request completion times */
+#define VR_BATCH_SHORTEST_FIRST 4u /* run scene groups in ascending order of paths */
+
+/* Enqueues the batch on `stream` (NULL = context stream).  A captured graph
+ * stays valid until the next render call on the context. */
+int vr_render_batch_async(vr_ctx* ctx, const vr_request* requests, uint32_t count,
+                          const vr_tuning* tuning, uint32_t flags, void* stream);
+/* Launches the last captured batch again (same targets, same random streams). */
+int vr_batch_relaunch_async(vr_ctx* ctx, void* stream);
+/* Per-request completion times of the last VR_BATCH_TIMESTAMPS batch, in ms
+ * after the batch started on the device (waits for the batch). */
+int vr_batch_latencies(vr_ctx* ctx, double* ms, uint32_t count);
+/* Statistics of the last batch (waits for it). */
+int vr_batch_last_stats(vr_ctx* ctx, vr_render_stats* stats);
+/* Times every candidate (chunk_paths x block) on the batch as a captured
+ * graph, `reps` timed launches each, and returns the fastest.  The targets
+ * are written during tuning: clear them afterwards. */
+int vr_batch_autotune(vr_ctx* ctx, const vr_request* requests, uint32_t count, uint32_t flags,
+                      uint32_t reps, void* stream, vr_tuning* best, float* best_ms);
 
 /* ---- post-processing (shaders/post_process.glsl:23-49) ----------------- */
 

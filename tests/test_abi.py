@@ -28,6 +28,8 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(N.vr_material) == 4 + 12 + 4 + 4 + 4
     assert C.sizeof(N.vr_trace_desc) == 16
     assert C.sizeof(N.vr_render_desc) == 4 * 10 + 8 + 8
+    assert C.sizeof(N.vr_request) == 8 + C.sizeof(N.vr_render_desc) + 8
+    assert C.sizeof(N.vr_tuning) == 8
 
 
 def test_errors_do_not_unwind():

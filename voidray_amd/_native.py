@@ -23,6 +23,10 @@ VR_RENDER_ROWS_BY_HEIGHT = 1
 VR_RENDER_ENV_IS = 2
 VR_RENDER_RUSSIAN_ROULETTE = 4
 VR_RENDER_ALPHA_PER_CALL = 8
+VR_RENDER_OVERWRITE = 16
+VR_BATCH_GRAPH = 1
+VR_BATCH_TIMESTAMPS = 2
+VR_BATCH_SHORTEST_FIRST = 4
 VR_TONEMAP_NONE, VR_TONEMAP_ACES, VR_TONEMAP_REINHARD, VR_TONEMAP_FILMIC, VR_TONEMAP_UNCHARTED2 = range(5)
 
 F3 = C.c_float * 3
@@ -95,6 +99,14 @@ class vr_post_desc(C.Structure):
                 ("exposure", C.c_float), ("tonemap", C.c_int32), ("clip_to_1024", C.c_uint32)]
 
 
+class vr_request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("desc", vr_render_desc), ("d_rgba", C.c_void_p)]
+
+
+class vr_tuning(C.Structure):
+    _fields_ = [("chunk_paths", C.c_uint32), ("block", C.c_uint32)]
+
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -112,6 +124,13 @@ SIGNATURES = {
     "vr_render": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats)]),
     "vr_render_device_async": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats), _P]),
     "vr_render_last_stats": (C.c_int, [_P, C.POINTER(vr_render_stats)]),
+    "vr_render_batch_async": (C.c_int, [_P, C.POINTER(vr_request), C.c_uint32, C.POINTER(vr_tuning), C.c_uint32,
+                                        _P]),
+    "vr_batch_relaunch_async": (C.c_int, [_P, _P]),
+    "vr_batch_latencies": (C.c_int, [_P, C.POINTER(C.c_double), C.c_uint32]),
+    "vr_batch_last_stats": (C.c_int, [_P, C.POINTER(vr_render_stats)]),
+    "vr_batch_autotune": (C.c_int, [_P, C.POINTER(vr_request), C.c_uint32, C.c_uint32, C.c_uint32, _P,
+                                    C.POINTER(vr_tuning), C.POINTER(C.c_float)]),
     "vr_tonemap": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P]),
     "vr_tonemap_device_async": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P, _P]),
 }
@@ -129,6 +148,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise VoidrayError(f"{LIB_PATH} is missing: build it with `python -m voidray_amd.build`")
+        # PyTorch-ROCm bundles its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Whichever loads first serves the whole process;
+        # if ours came first, torch.cuda would find no device.  Load torch's
+        # first when torch is installed, so device tensors and this library
+        # share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

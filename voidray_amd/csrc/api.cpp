@@ -75,7 +75,9 @@ int vr_create(int device, vr_ctx** out) {
 
 int vr_destroy(vr_ctx* ctx) {
     if (!ctx) return 0;
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();  // batches may run on caller streams
+    free_render_state(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -174,8 +176,9 @@ int vr_render_device_async(vr_scene* s, const vr_render_desc* desc, float* d_rgb
                            void* stream) {
     return guarded([&] {
         if (!s || !desc || !d_rgba) return fail(VR_ERR_INVALID, "null argument");
+        (void)stats;
         hipStream_t st = stream ? (hipStream_t)stream : s->ctx->stream;
-        return render_device(s, desc, d_rgba, stats, st);
+        return render_device(s, desc, d_rgba, st);
     });
 }
 
@@ -191,7 +194,7 @@ int vr_render(vr_scene* s, const vr_render_desc* desc, float* rgba, vr_render_st
         VR_HIP_CHECK(hipEventCreate(&e0));
         VR_HIP_CHECK(hipEventCreate(&e1));
         VR_HIP_CHECK(hipEventRecord(e0, st));
-        rc = render_device(s, desc, (float*)g_dev_stage.p, stats, st);
+        rc = render_device(s, desc, (float*)g_dev_stage.p, st);
         if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -206,7 +209,7 @@ int vr_render(vr_scene* s, const vr_render_desc* desc, float* rgba, vr_render_st
         (void)hipEventDestroy(e1);
         if (stats) {
             stats->kernel_ms = ms;
-            render_finish_stats(s, stats);
+            return render_finish_stats(s->ctx, stats);
         }
         return 0;
     });
@@ -214,10 +217,51 @@ int vr_render(vr_scene* s, const vr_render_desc* desc, float* rgba, vr_render_st
 
 int vr_render_last_stats(vr_scene* s, vr_render_stats* stats) {
     if (!s || !stats) return fail(VR_ERR_INVALID, "null argument");
-    VR_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
     stats->kernel_ms = 0.0;
-    render_finish_stats(s, stats);
-    return 0;
+    return render_finish_stats(s->ctx, stats);
+}
+
+int vr_render_batch_async(vr_ctx* ctx, const vr_request* requests, uint32_t count, const vr_tuning* tuning,
+                          uint32_t flags, void* stream) {
+    return guarded([&] {
+        if (!ctx || (count && !requests)) return fail(VR_ERR_INVALID, "null argument");
+        VR_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        return render_batch(ctx, requests, count, tuning, flags, st);
+    });
+}
+
+int vr_batch_relaunch_async(vr_ctx* ctx, void* stream) {
+    return guarded([&] {
+        if (!ctx) return fail(VR_ERR_INVALID, "null argument");
+        VR_HIP_CHECK(hipSetDevice(ctx->device));
+        return relaunch_batch(ctx, stream ? (hipStream_t)stream : ctx->stream);
+    });
+}
+
+int vr_batch_latencies(vr_ctx* ctx, double* ms, uint32_t count) {
+    return guarded([&] {
+        if (!ctx || (count && !ms)) return fail(VR_ERR_INVALID, "null argument");
+        return batch_latencies(ctx, ms, count);
+    });
+}
+
+int vr_batch_last_stats(vr_ctx* ctx, vr_render_stats* stats) {
+    return guarded([&] {
+        if (!ctx || !stats) return fail(VR_ERR_INVALID, "null argument");
+        stats->kernel_ms = 0.0;
+        return render_finish_stats(ctx, stats);
+    });
+}
+
+int vr_batch_autotune(vr_ctx* ctx, const vr_request* requests, uint32_t count, uint32_t flags, uint32_t reps,
+                      void* stream, vr_tuning* best, float* best_ms) {
+    return guarded([&] {
+        if (!ctx || !best || (count && !requests)) return fail(VR_ERR_INVALID, "null argument");
+        VR_HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+        return autotune_batch(ctx, requests, count, flags, reps, st, best, best_ms);
+    });
 }
 
 int vr_tonemap_device_async(vr_ctx* ctx, const vr_post_desc* desc, const float* d_src, float* d_dst, void* stream) {

@@ -126,9 +126,14 @@ struct SceneView {
 
 }  // namespace vr
 
+namespace vr {
+struct RenderState;
+}
+
 struct vr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    vr::RenderState* render = nullptr;  // batch planning, render scratch, captured graph (render.hip)
 };
 
 struct vr_scene {
@@ -147,13 +152,7 @@ struct vr_scene {
     hipEvent_t trace_ev[2] = {nullptr, nullptr};  // around the wide kernel of the last vr_trace*
     bool trace_timed = false;
     uint32_t last_fallbacks = 0;
-    // host copies used by the render driver
     uint32_t num_prims = 0;
-    // render scratch (grown on demand)
-    void* render_scratch = nullptr;
-    size_t render_scratch_bytes = 0;
-    uint64_t* render_stats_dev = nullptr;   // [segments, samples, misses, fallbacks, capped]
-    uint64_t* render_stats_host = nullptr;  // pinned copy, valid after the stream completes
 };
 
 namespace vr {

@@ -2,8 +2,19 @@
 #include "internal.h"
 
 namespace vr {
-int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, vr_render_stats* stats, hipStream_t st);
-// Completes the counters of the last render after the stream has been synchronised.
-void render_finish_stats(vr_scene* s, vr_render_stats* stats);
+// One request: a batch of one.
+int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, hipStream_t st);
+// Plans, uploads and enqueues a batch (optionally captured into a graph and launched).
+int render_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning* tuning, uint32_t flags,
+                 hipStream_t st);
+// Launches the last captured batch again.
+int relaunch_batch(vr_ctx* ctx, hipStream_t st);
+// Counters of the last render or batch on the context (waits for its read-back).
+int render_finish_stats(vr_ctx* ctx, vr_render_stats* stats);
+// Completion time of each request of the last time-stamped batch, ms after its start.
+int batch_latencies(vr_ctx* ctx, double* ms, uint32_t n);
+int autotune_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, uint32_t flags, uint32_t reps,
+                   hipStream_t st, vr_tuning* best, float* best_ms);
+void free_render_state(vr_ctx* ctx);
 int tonemap_device(const vr_post_desc* desc, const float* d_src, float* d_dst, hipStream_t st);
 }  // namespace vr

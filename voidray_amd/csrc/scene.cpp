@@ -349,19 +349,13 @@ void free_scene(vr_scene* s) {
     s->allocations.clear();
     if (s->fallback_list) (void)hipFree(s->fallback_list);
     if (s->counters) (void)hipFree(s->counters);
-    if (s->render_scratch) (void)hipFree(s->render_scratch);
     if (s->spill) (void)hipFree(s->spill);
     for (auto& e : s->trace_ev)
         if (e) (void)hipEventDestroy(e);
     s->trace_ev[0] = s->trace_ev[1] = nullptr;
-    if (s->render_stats_dev) (void)hipFree(s->render_stats_dev);
-    if (s->render_stats_host) (void)hipHostFree(s->render_stats_host);
-    s->render_stats_dev = nullptr;
-    s->render_stats_host = nullptr;
     s->spill = nullptr;
     s->fallback_list = nullptr;
     s->counters = nullptr;
-    s->render_scratch = nullptr;
 }
 
 }  // namespace vr

@@ -114,6 +114,58 @@ class SceneAcceleration:
         return st
 
 
+class BatchRenderer:
+    """Serves many render requests in one launch (vr_render_batch_async).
+
+    The reference renders one frame per RenderThread::one_shot call
+    (render/renderer.rs:67-107).  Here a request is (SceneAcceleration,
+    vr_render_desc, device target pointer); requests on one scene share
+    wavefront chunks, and with `graph=True` the whole batch is captured into a
+    hipGraph that `relaunch` replays.  All calls go to one stream."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    @staticmethod
+    def _array(requests):
+        arr = (N.vr_request * max(1, len(requests)))()
+        for k, (accel, desc, d_rgba) in enumerate(requests):
+            arr[k].scene = accel.handle.value
+            arr[k].desc = desc
+            arr[k].d_rgba = int(d_rgba)
+        return arr
+
+    def submit(self, requests, tuning=None, graph=True, timestamps=False, shortest_first=False, stream=None):
+        flags = ((N.VR_BATCH_GRAPH if graph else 0) | (N.VR_BATCH_TIMESTAMPS if timestamps else 0)
+                 | (N.VR_BATCH_SHORTEST_FIRST if shortest_first else 0))
+        t = N.vr_tuning(*tuning) if tuning is not None else None
+        N.check(N.lib().vr_render_batch_async(self.ctx.handle, self._array(requests), len(requests),
+                                              C.byref(t) if t is not None else None, flags, stream))
+
+    def relaunch(self, stream=None):
+        N.check(N.lib().vr_batch_relaunch_async(self.ctx.handle, stream))
+
+    def latencies(self, n):
+        """Completion time of each request (ms after the batch started on the device)."""
+        out = np.zeros(n, np.float64)
+        N.check(N.lib().vr_batch_latencies(self.ctx.handle, out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out
+
+    def stats(self):
+        st = N.vr_render_stats()
+        N.check(N.lib().vr_batch_last_stats(self.ctx.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in N.vr_render_stats._fields_}
+
+    def autotune(self, requests, reps=2, shortest_first=False, stream=None):
+        """Returns ((chunk_paths, block), ms) of the fastest candidate; the targets are overwritten."""
+        best = N.vr_tuning()
+        ms = C.c_float()
+        flags = N.VR_BATCH_SHORTEST_FIRST if shortest_first else 0
+        N.check(N.lib().vr_batch_autotune(self.ctx.handle, self._array(requests), len(requests), flags, reps,
+                                          stream, C.byref(best), C.byref(ms)))
+        return (best.chunk_paths, best.block), ms.value
+
+
 def make_render_desc(width, height, samples, total_samples, settings=None, sample_offset=0, seed=0, request=0,
                      flags=N.VR_RENDER_ALPHA_PER_CALL, rr_depth=0, engine=N.VR_ENGINE_WIDE):
     from .scene import RenderSettings
