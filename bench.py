@@ -108,6 +108,95 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
     return out
 
 
+def _pct(x, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(x), q)), 3)
+
+
+def _serving_leg(ctx, torch, stream):
+    """64 render requests (resolutions 256^2..1920x1080, 1-32 spp, own seeds)
+    over eight scenes of 10k-2M triangles; depth 16, RR from 4, env IS + MIS.
+    Batched: one captured hipGraph launch with the autotuned chunk/block size
+    (tuned on a proxy with spp / 8); completion time of each request from
+    device time stamps.  Sequential: the same requests one render call each,
+    shortest first, back to back on the same stream; completion times from HIP
+    events.  Latency = completion time with all requests submitted at t = 0.
+    The per-image checksums of both runs must agree (same random streams)."""
+    from voidray_amd import _native as N
+    from voidray_amd import workloads as W
+    from voidray_amd.device import BatchRenderer, make_render_desc
+    from voidray_amd.scene import RenderSettings
+    sp = stream.cuda_stream
+    t0 = time.time()
+    pool = W.serving_scene_pool()
+    accels = [s.build_acceleration(ctx) for s in pool]
+    build_s = time.time() - t0
+    reqs = W.serving_requests()
+    flags = (N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+             | N.VR_RENDER_OVERWRITE)
+    targets, batch, proxy = [], [], []
+    for i, (si, w, h, spp, seed) in enumerate(reqs):
+        st = RenderSettings(total_samples=spp, max_bounces=16, firefly_clamp=float("inf"))
+        t = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+        targets.append(t)
+        batch.append((accels[si], make_render_desc(w, h, spp, spp, st, seed=seed, request=i, flags=flags,
+                                                   rr_depth=4), t.data_ptr()))
+        ps = max(1, spp // 8)
+        proxy.append((accels[si], make_render_desc(w, h, ps, ps, st, seed=seed, request=i, flags=flags,
+                                                   rr_depth=4), t.data_ptr()))
+    samples = sum(w * h * spp for _, w, h, spp, _ in reqs)
+    torch.cuda.synchronize()
+    br = BatchRenderer(ctx)
+    t1 = time.time()
+    (chunk, block), tune_ms = br.autotune(proxy, reps=1, shortest_first=True, stream=sp)
+    tune_s = time.time() - t1
+    # batched: capture + first launch (warm-up), then one timed relaunch
+    t1 = time.time()
+    br.submit(batch, tuning=(chunk, block), graph=True, timestamps=True, shortest_first=True, stream=sp)
+    br.stats()
+    capture_first_s = time.time() - t1
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    br.relaunch(stream=sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1)
+    lat_b = br.latencies(len(reqs))
+    bst = br.stats()
+    sums_b = torch.stack([t.double().sum() for t in targets]).cpu()
+    nonfinite = int(sum((~torch.isfinite(t)).sum().item() for t in targets))
+    # sequential: one request per call on the same stream, shortest first
+    order = sorted(range(len(reqs)), key=lambda k: reqs[k][1] * reqs[k][2] * reqs[k][3])
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(reqs) + 1)]
+    evs[0].record(stream)
+    for k, i in enumerate(order):
+        acc, desc, ptr = batch[i]
+        acc.render_device(ptr, desc, stream=sp)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    lat_s = [evs[0].elapsed_time(e) for e in evs[1:]]
+    svc_s = [evs[k].elapsed_time(evs[k + 1]) for k in range(len(reqs))]
+    seq_ms = lat_s[-1]
+    sums_s = torch.stack([t.double().sum() for t in targets]).cpu()
+    return {
+        "requests": len(reqs), "samples": samples, "scenes_triangles": [s.num_triangles for s in pool],
+        "scene_build_s": round(build_s, 2), "settings": "depth 16, RR from 4, env IS + MIS",
+        "autotune": {"chunk_paths": chunk, "block": block, "proxy_ms": round(tune_ms, 2),
+                     "seconds": round(tune_s, 1), "proxy": "same requests with spp // 8"},
+        "batched": {"ms": round(batch_ms, 2), "samples_per_s": round(samples / (batch_ms * 1e-3), 1),
+                    "mrays_per_s": round(bst["segments"] / (batch_ms * 1e3), 2),
+                    "p50_ms": _pct(lat_b, 50), "p99_ms": _pct(lat_b, 99),
+                    "capture_and_first_launch_s": round(capture_first_s, 2)},
+        "sequential": {"ms": round(seq_ms, 2), "samples_per_s": round(samples / (seq_ms * 1e-3), 1),
+                       "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
+                       "service_p50_ms": _pct(svc_s, 50), "service_p99_ms": _pct(svc_s, 99)},
+        "batched_over_sequential": round(seq_ms / batch_ms, 3),
+        "images_equal_checksums": bool(torch.equal(sums_b, sums_s)),
+        "nonfinite_values": nonfinite,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +206,8 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-integration legs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving leg")
+    ap.add_argument("--only-serving", action="store_true", help="run the batched-serving leg alone (prints it)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 closest-hit queries of the workload, then a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
@@ -134,6 +225,11 @@ def main():
 
     dev = torch.device("cuda:0")
     ctx = Context(0)
+    if args.only_serving:
+        st = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(st)
+        print(json.dumps({"serving": _serving_leg(ctx, torch, st)}), flush=True)
+        return
     scene = W.random_triangle_scene(args.tris)
     o, d = W.microbench_rays(args.rays)
     t0 = time.time()
@@ -206,6 +302,7 @@ def main():
     achieved = BYTES_PER_RAY_CLOSEST * n / (kernel_ms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
+    serving = None if args.no_serving else _serving_leg(ctx, torch, stream)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),
@@ -239,6 +336,7 @@ def main():
             "wall_s": round(wall, 3),
             "gpu_over_cpu": (round(mrays / cpu["value"], 1) if cpu else None),
             "render": renders,
+            "serving": serving,
         },
     }
     print(json.dumps(out), flush=True)

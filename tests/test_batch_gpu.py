@@ -131,17 +131,27 @@ def test_shortest_first_orders_completion(gpu_ctx, scenes):
     reqs = _requests()
     targets = _targets(reqs)
     br = BatchRenderer(gpu_ctx)
-    announce("batch time stamps, shortest scene group first")
+    announce("batch time stamps, shortest group first")
     br.submit(_batch(scenes, reqs, targets), tuning=(1 << 22, 256), timestamps=True, shortest_first=True)
     lat = br.latencies(len(reqs))
+
+    def size_class(p):  # paths in [4^c, 4^(c+1))
+        c = 0
+        while p >> (2 * (c + 1)):
+            c += 1
+        return c
+
+    # groups: (scene, engine, size class), run in ascending order of their paths
+    key = [(si, d.engine, size_class(d.width * d.height * d.samples)) for si, d in reqs]
     paths = {}
     for k, (si, d) in enumerate(reqs):
-        key = (si, d.engine)
-        paths[key] = paths.get(key, 0) + d.width * d.height * d.samples
+        paths[key[k]] = paths.get(key[k], 0) + d.width * d.height * d.samples
     order = sorted(paths, key=lambda g: paths[g])
-    done = {g: max(lat[k] for k, (si, d) in enumerate(reqs) if (si, d.engine) == g) for g in paths}
+    done = {g: max(lat[k] for k in range(len(reqs)) if key[k] == g) for g in paths}
+    start = {g: min(lat[k] for k in range(len(reqs)) if key[k] == g) for g in paths}
     for a, b in zip(order, order[1:]):
-        assert done[a] <= done[b], (a, b, done)
+        if paths[a] < paths[b]:
+            assert done[a] <= start[b], (a, b, done, start)
 
 
 def test_autotune_picks_a_candidate(gpu_ctx, scenes):
@@ -150,7 +160,7 @@ def test_autotune_picks_a_candidate(gpu_ctx, scenes):
     br = BatchRenderer(gpu_ctx)
     announce("batch autotune")
     (chunk, block), ms = br.autotune(_batch(scenes, reqs, targets), reps=1)
-    assert chunk in (1 << 20, 1 << 21, 1 << 22, 1 << 23) and block in (128, 256, 512)
+    assert chunk in (1 << 21, 1 << 22, 1 << 23, 1 << 24) and block in (128, 256, 512)
     assert ms > 0.0
     # the tuned configuration still renders every request exactly
     want = _standalone(scenes, reqs)

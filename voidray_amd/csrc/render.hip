@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 
 #include "render.h"
 #include "shade.h"
@@ -504,7 +505,11 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
     R.tune = tune;
     R.tune.chunk_paths = chunk_paths;
     R.tune.block = block;
-    std::map<std::pair<vr_scene*, uint32_t>, size_t> gidx;
+    // group key: scene, engine and, when shortest-first, the request's size
+    // class (paths in [4^c, 4^(c+1))), so that small requests on every scene
+    // run before large ones instead of waiting for their scene's whole group
+    std::map<std::tuple<vr_scene*, uint32_t, uint32_t>, size_t> gidx;
+    const bool sjf = (flags & VR_BATCH_SHORTEST_FIRST) != 0;
     for (uint32_t i = 0; i < n; ++i) {
         const vr_request& it = items[i];
         if (!it.scene || !it.d_rgba) return fail(VR_ERR_INVALID, "request without scene or target");
@@ -528,7 +533,11 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         r.clamp = d.firefly_clamp;
         r.ext = is_ext(d) ? 1u : 0u;
         R.reqs.push_back(r);
-        auto key = std::make_pair(it.scene, d.engine);
+        uint64_t rp = (uint64_t)d.width * d.height * d.samples;
+        uint32_t size_class = 0;
+        if (sjf)
+            while (rp >> (2 * (size_class + 1))) ++size_class;
+        auto key = std::make_tuple(it.scene, d.engine, size_class);
         auto f = gidx.find(key);
         if (f == gidx.end()) {
             Group g;
@@ -545,9 +554,14 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         g.max_depth = std::max(g.max_depth, d.max_bounces);
         g.paths += (uint64_t)d.width * d.height * d.samples;
     }
-    if (flags & VR_BATCH_SHORTEST_FIRST)
+    if (flags & VR_BATCH_SHORTEST_FIRST) {
         std::stable_sort(R.groups.begin(), R.groups.end(),
                          [](const Group& a, const Group& b) { return a.paths < b.paths; });
+        auto paths = [&](uint32_t i) { return (uint64_t)R.reqs[i].W * R.reqs[i].H * R.reqs[i].spp; };
+        for (Group& g : R.groups)
+            std::stable_sort(g.reqs.begin(), g.reqs.end(),
+                             [&](uint32_t a, uint32_t b) { return paths(a) < paths(b); });
+    }
     for (Group& g : R.groups) {
         // cut the group's pixels into chunks of <= chunk_paths paths
         std::vector<uint32_t> last_chunk;
@@ -849,7 +863,7 @@ int batch_latencies(vr_ctx* ctx, double* ms, uint32_t n) {
 // relaunches after one warm-up launch) and returns the fastest.
 int autotune_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, uint32_t flags, uint32_t reps,
                    hipStream_t st, vr_tuning* best, float* best_ms) {
-    static const uint32_t chunks[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    static const uint32_t chunks[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     static const uint32_t blocks[] = {128, 256, 512};
     uint32_t max_spp = 1;
     for (uint32_t i = 0; i < n; ++i) max_spp = std::max(max_spp, items[i].desc.samples);

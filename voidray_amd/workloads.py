@@ -13,6 +13,9 @@
 * terrain_spheres   — 1920x1080 path-integration scene: fBm heightfield,
                       200 UV spheres, mixed materials, procedural textures and
                       HDR sky with a sun lobe.
+* serving_*         — batched serving: 64 requests (resolutions from 256^2 to
+                      1920x1080, 1-32 spp, own seeds) over eight scenes of 10k
+                      to 2M triangles from the two generators above.
 """
 import numpy as np
 
@@ -346,3 +349,61 @@ def cornell_zero_normal_quads(seed=1):
     scene.camera = Camera.look_at((278 * s, 278 * s, -800 * s), (278 * s, 278 * s, 0.0), (0.0, 1.0, 0.0),
                                   degrees_to_radians(40.0))
     return scene
+
+
+# -- batched serving: 64 requests over a pool of scenes -------------------------
+SERVING_RESOLUTIONS = ((256, 256), (512, 512), (1280, 720), (1920, 1080))
+
+
+def lit_random_triangle_scene(num_tris, seed=2, sky=(512, 256)):
+    """The traversal-microbench geometry with mixed materials, a camera in front
+    of the unit cube and an HDR sky, so that it can be rendered."""
+    rng = _rng(seed + 100)
+    scene = Scene.empty()
+    tris = random_triangles(num_tris, seed)
+    third = num_tris // 3
+    parts = [(0, third), (third, 2 * third), (2 * third, num_tris)]
+    mats = [Materials.lambertian(rng.uniform(0.2, 0.8, 3)),
+            Materials.metal(rng.uniform(0.5, 0.95, 3), float(rng.uniform(0.0, 1.0))), Materials.dielectric(1.5)]
+    for (a, b), m in zip(parts, mats):
+        if b <= a:
+            continue
+        pos = tris.positions[3 * a:3 * b]
+        idx = np.arange(3 * (b - a), dtype=np.uint32).reshape(-1, 3)
+        # vertex normals = face normals (a mesh without normals has zero normals, which shade to NaN)
+        p = pos.reshape(-1, 3, 3).astype(np.float64)
+        fn = _normalize_rows(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]))
+        nrm = np.repeat(fn, 3, axis=0).astype(f32)
+        scene.add_object(scene.add_material(m), scene.add_mesh(Mesh(pos, idx, nrm)))
+    scene.environment = Environments.hdri(procedural_sky(*sky, seed=seed + 5, sun_peak=1e3, sun_sharpness=500.0))
+    scene.camera = Camera.look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), degrees_to_radians(45.0))
+    return scene
+
+
+def serving_scene_pool(seed=7):
+    """Eight scenes of 10k to 2M triangles from the microbench and terrain
+    generators (smaller textures and skies than the 1080p terrain scene)."""
+    specs = [("tris", 10_000), ("terrain", (72, 20, (16, 8))), ("tris", 100_000), ("terrain", (224, 50, (32, 16))),
+             ("tris", 400_000), ("terrain", (448, 100, (32, 16))), ("tris", 1_000_000),
+             ("terrain", (990, 200, (16, 8)))]
+    pool = []
+    for k, (kind, arg) in enumerate(specs):
+        if kind == "tris":
+            pool.append(lit_random_triangle_scene(arg, seed=seed + k))
+        else:
+            grid, ns, res = arg
+            pool.append(terrain_spheres(grid=grid, n_spheres=ns, sphere_res=res, tex_size=512, env_size=(512, 256),
+                                        sun_peak=1e4, seed=seed + k))
+    return pool
+
+
+def serving_requests(n=64, pool_size=8, seed=7):
+    """n requests: (scene index, width, height, spp, seed).  Resolution uniform
+    over SERVING_RESOLUTIONS, spp uniform in 1..32, scene uniform over the
+    pool, request i seeded with seed + i."""
+    rng = _rng(seed)
+    out = []
+    for i in range(n):
+        w, h = SERVING_RESOLUTIONS[int(rng.integers(len(SERVING_RESOLUTIONS)))]
+        out.append((int(rng.integers(pool_size)), w, h, int(rng.integers(1, 33)), seed + i))
+    return out
