@@ -282,6 +282,94 @@ int vr_batch_last_stats(vr_ctx* ctx, vr_render_stats* stats);
 int vr_batch_autotune(vr_ctx* ctx, const vr_request* requests, uint32_t count, uint32_t flags,
                       uint32_t reps, void* stream, vr_tuning* best, float* best_ms);
 
+/* ---- learned radiance cache ---------------------------------------------
+ * No counterpart in the reference (its integrator traces every path to the
+ * end, core/tracer.rs:9-56).  A multiresolution hash grid on position (16
+ * levels x 2^log2_table entries x 2 FP16 features) plus one-blob encodings of
+ * the outgoing direction, shading normal and roughness feed an MLP
+ * 128 -> 64 -> 64 -> 64 -> 64 -> 3 (ReLU, no biases) that predicts outgoing
+ * radiance.  BF16 weights and activations on MFMA with FP32 accumulation
+ * (precision VR_CACHE_FP32: an all-FP32 debug path on the vector ALUs).
+ * Trained online with Adam on a relative L2 loss against path-suffix radiance
+ * produced by the tracer itself; queried in place of the rest of a path. */
+
+typedef struct vr_cache vr_cache;
+
+#define VR_CACHE_BF16 0
+#define VR_CACHE_FP32 1
+#define VR_CACHE_LEVELS 16
+#define VR_CACHE_MLP_PARAMS 21504 /* 64x128 + 3 x 64x64 + 16x64 (rows 3..15 of the last layer unused) */
+
+typedef struct vr_cache_desc {
+    uint32_t log2_table;      /* entries per level = 2^log2_table (19 for the full cache) */
+    uint32_t base_resolution; /* coarsest grid resolution (cells per axis) */
+    uint32_t max_resolution;  /* finest grid resolution */
+    uint32_t precision;       /* VR_CACHE_* */
+    float bbox_lo[3], bbox_hi[3]; /* positions are normalised by the largest extent of this box */
+    float lr_grid, lr_mlp;    /* Adam learning rates */
+    float beta1, beta2, eps;
+    uint64_t seed;            /* parameter initialisation */
+} vr_cache_desc;
+
+typedef struct vr_cache_info {
+    uint32_t levels, table_size, precision, pad;
+    uint32_t resolution[VR_CACHE_LEVELS];
+    float origin[3];
+    float inv_extent;         /* p' = (p - origin) * inv_extent, clamped to [0, 1] */
+    uint64_t grid_params;     /* levels * table_size * 2 */
+    uint64_t steps;           /* Adam steps taken */
+} vr_cache_info;
+
+typedef struct vr_cache_frame {
+    uint32_t train_paths;     /* training paths traced per frame (their vertices become samples) */
+    uint32_t train_vertices;  /* vertices per training path recorded as samples (<= 8) */
+    uint32_t cache_depth;     /* render paths query the cache at this vertex (2 = after bounce 2) */
+    uint32_t train_steps;     /* Adam steps per frame over the frame's samples */
+    uint32_t frame;           /* frame index: selects the training pixels and random streams */
+    uint32_t pad;
+} vr_cache_frame;
+
+typedef struct vr_cache_stats {
+    double loss;              /* mean relative L2 loss of the last training step */
+    uint64_t train_samples;   /* samples of the last frame */
+    uint64_t queries;         /* cache queries of the last frame */
+    double train_ms, render_ms;
+} vr_cache_stats;
+
+int vr_cache_create(vr_ctx* ctx, const vr_cache_desc* desc, vr_cache** out);
+int vr_cache_destroy(vr_cache* cache);
+int vr_cache_get_info(const vr_cache* cache, vr_cache_info* info);
+/* Parameters as the forward pass uses them (BF16 weights / FP16 grid, as f32;
+ * FP32 precision: the master copies).  mlp: VR_CACHE_MLP_PARAMS floats, rows
+ * [out][in] per layer; grid: [level][entry][feature]. */
+int vr_cache_get_params(vr_cache* cache, float* mlp, float* grid);
+/* Sets the master copies (and their rounded forward copies); resets Adam. */
+int vr_cache_set_params(vr_cache* cache, const float* mlp, const float* grid);
+/* Device arrays: pos, dir (outgoing, unit), nrm (unit) n*3 f32, rough n f32.
+ * out: n*3 f32 predicted radiance. */
+int vr_cache_infer_async(vr_cache* cache, uint32_t n, const float* d_pos, const float* d_dir,
+                         const float* d_nrm, const float* d_rough, float* d_out, void* stream);
+/* Gradients of the mean relative L2 loss sum_c (y_c - t_c)^2 / (y_c^2 + 0.01)
+ * / (3n) (denominator not differentiated) w.r.t. the forward parameters,
+ * without an update.  d_mlp_grad: VR_CACHE_MLP_PARAMS f32; d_grid_grad:
+ * grid_params f32; d_loss: 1 f32.  Any output may be NULL. */
+int vr_cache_gradients_async(vr_cache* cache, uint32_t n, const float* d_pos, const float* d_dir,
+                             const float* d_nrm, const float* d_rough, const float* d_target,
+                             float* d_mlp_grad, float* d_grid_grad, float* d_loss, void* stream);
+/* One Adam step on the given samples. */
+int vr_cache_train_async(vr_cache* cache, uint32_t n, const float* d_pos, const float* d_dir,
+                         const float* d_nrm, const float* d_rough, const float* d_target, void* stream);
+/* One frame of the cached renderer: trace `train_paths` full paths of the
+ * extended estimator (flags ENV_IS/RR), record their first `train_vertices`
+ * vertices with path-suffix radiance as samples, take `train_steps` Adam
+ * steps, then render desc->samples paths per pixel that stop at vertex
+ * `cache_depth` and add throughput x cache(vertex).  Accumulates into d_rgba
+ * like vr_render_device_async. */
+int vr_render_cached_async(vr_scene* scene, vr_cache* cache, const vr_render_desc* desc,
+                           const vr_cache_frame* frame, float* d_rgba, void* stream);
+/* Statistics of the last frame (waits for it). */
+int vr_cache_last_stats(vr_cache* cache, vr_cache_stats* stats);
+
 /* ---- post-processing (shaders/post_process.glsl:23-49) ----------------- */
 
 #define VR_TONEMAP_NONE 0

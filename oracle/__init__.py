@@ -60,8 +60,69 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_tex_sample_f32.argtypes = [_P, C.c_float, C.c_float, _P]
         L.orc_env_sample_f32.argtypes = [C.c_uint32, C.c_uint32, _P, _P, _P]
+        L.orc_nrc_forward.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P, C.c_float, _P, _P, _P, C.c_int]
+        L.orc_nrc_gradients.restype = C.c_double
+        L.orc_nrc_gradients.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P, C.c_float, _P, _P, _P, _P,
+                                        C.c_int, C.c_int]
+        L.orc_nrc_corners.argtypes = [_P, C.c_int, _P, C.c_uint32, _P, C.c_float, _P, _P]
         _lib = L
     return _lib
+
+
+class OracleCache:
+    """FP64 restatement of the learned radiance cache's network
+    (oracle/nrc_oracle.cpp): encode, forward, relative L2 loss, gradients.
+    `layout`: resolution (16 ints), log2_table, origin (3), inv_extent — the
+    values vr_cache_get_info reports; `mlp` (21504) and `grid` (16*2^log2*2):
+    the parameters as the forward pass uses them."""
+
+    def __init__(self, layout, mlp, grid):
+        self.res = np.ascontiguousarray(layout["resolution"], np.uint32)
+        self.log2 = int(layout["log2_table"])
+        self.origin = np.ascontiguousarray(layout["origin"], np.float32)
+        self.inv_extent = float(np.float32(layout["inv_extent"]))
+        self.mlp = np.ascontiguousarray(mlp, np.float64)
+        self.grid = np.ascontiguousarray(grid, np.float64)
+        assert self.mlp.size == 21504 and self.grid.size == (16 << self.log2) * 2
+
+    @staticmethod
+    def _in(pos, dirs, nrm, rough):
+        f = lambda a, k: np.ascontiguousarray(a, np.float32).reshape(-1, k) if k > 1 else \
+            np.ascontiguousarray(a, np.float32).reshape(-1)
+        return f(pos, 3), f(dirs, 3), f(nrm, 3), f(rough, 1)
+
+    def forward(self, pos, dirs, nrm, rough, bf16=False):
+        """Predicted radiance (n, 3).  bf16: the rounding model of the BF16 kernels
+        (inputs, activations and deltas rounded to bf16, FP64 accumulation)."""
+        p, d, n, r = self._in(pos, dirs, nrm, rough)
+        out = np.zeros((len(p), 3), np.float64)
+        lib().orc_nrc_forward(len(p), p.ctypes.data, d.ctypes.data, n.ctypes.data, r.ctypes.data,
+                              self.res.ctypes.data, self.log2, self.origin.ctypes.data, self.inv_extent,
+                              self.mlp.ctypes.data, self.grid.ctypes.data, out.ctypes.data, 1 if bf16 else 0)
+        return out
+
+    def gradients(self, pos, dirs, nrm, rough, target, abs_mode=False, bf16=False):
+        """(loss, MLP gradient, grid gradient).  abs_mode: every weight, input and
+        loss derivative taken by absolute value — per entry, the sum of the
+        absolute values of its terms (the scale of its rounding error).  bf16:
+        the BF16 kernels' rounding model."""
+        p, d, n, r = self._in(pos, dirs, nrm, rough)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        gw = np.zeros(21504, np.float64)
+        gg = np.zeros(self.grid.size, np.float64)
+        loss = lib().orc_nrc_gradients(len(p), p.ctypes.data, d.ctypes.data, n.ctypes.data, r.ctypes.data,
+                                       t.ctypes.data, self.res.ctypes.data, self.log2, self.origin.ctypes.data,
+                                       self.inv_extent, self.mlp.ctypes.data, self.grid.ctypes.data, gw.ctypes.data,
+                                       gg.ctypes.data, 1 if abs_mode else 0, 1 if bf16 else 0)
+        return loss, gw, gg
+
+    def corners(self, pos, level):
+        p = np.ascontiguousarray(pos, np.float32).reshape(3)
+        idx = np.zeros(8, np.uint32)
+        w = np.zeros(8, np.float64)
+        lib().orc_nrc_corners(p.ctypes.data, level, self.res.ctypes.data, self.log2, self.origin.ctypes.data,
+                              self.inv_extent, idx.ctypes.data, w.ctypes.data)
+        return idx, w
 
 
 def philox(ctr, key):

@@ -8,13 +8,13 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-SRC = os.path.join(HERE, "voidray_oracle.cpp")
+SRC = [os.path.join(HERE, "voidray_oracle.cpp"), os.path.join(HERE, "nrc_oracle.cpp")]
 
 
 def build():
     tmp = LIB + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
-           "-fno-fast-math", "-mno-fma", SRC, "-o", tmp]
+           "-fno-fast-math", "-mno-fma", *SRC, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")

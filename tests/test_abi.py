@@ -30,6 +30,10 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(N.vr_render_desc) == 4 * 10 + 8 + 8
     assert C.sizeof(N.vr_request) == 8 + C.sizeof(N.vr_render_desc) + 8
     assert C.sizeof(N.vr_tuning) == 8
+    assert C.sizeof(N.vr_cache_desc) == 16 + 24 + 20 + 4 + 8
+    assert C.sizeof(N.vr_cache_info) == 16 + 64 + 16 + 16
+    assert C.sizeof(N.vr_cache_frame) == 24
+    assert C.sizeof(N.vr_cache_stats) == 40
 
 
 def test_errors_do_not_unwind():

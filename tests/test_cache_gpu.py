@@ -1,0 +1,188 @@
+"""Learned radiance cache on the GPU against the FP64 oracle (oracle/nrc_oracle.cpp).
+
+Bars (relative error = ||gpu - oracle|| / ||oracle|| over the whole array or layer):
+* FP32 debug path: forward, loss and every gradient <= 1e-6;
+* BF16 / MFMA path: forward and loss <= 2e-2; every gradient entry within
+  2^-5 of the sum of the absolute values of its terms (oracle abs mode):
+  BF16 keeps 8 significant bits of inputs, activations and deltas, and an
+  entry that sums many signed per-sample terms can lose most of them to
+  cancellation, so a norm-wise bar alone would measure conditioning, not the
+  kernels (the norm-wise errors are printed);
+* training on a smooth synthetic radiance field reduces the validation loss 10x;
+* the cached renderer produces finite images and trains on its own samples.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import announce
+from voidray_amd import _native as N
+from voidray_amd.device import RadianceCache
+
+pytestmark = pytest.mark.gpu
+
+OFF = [0, 8192, 12288, 16384, 20480, 21504]
+LO, HI = np.float32([-1.0, -0.5, 0.25]), np.float32([3.0, 3.5, 4.25])
+
+
+def _inputs(n, seed=1):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(LO, HI, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nr = rng.normal(size=(n, 3))
+    nr /= np.linalg.norm(nr, axis=1, keepdims=True)
+    rough = rng.uniform(0, 1, n).astype(np.float32)
+    target = rng.uniform(0.0, 3.0, (n, 3)).astype(np.float32)
+    return pos, d.astype(np.float32), nr.astype(np.float32), rough, target
+
+
+def _dev(*arrays):
+    import torch
+    out = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrays]
+    torch.cuda.synchronize()
+    return out
+
+
+def _cache(ctx, precision, log2=10):
+    c = RadianceCache(ctx, LO, HI, log2_table=log2, base_resolution=4, max_resolution=96, precision=precision, seed=3)
+    rng = np.random.default_rng(7)
+    mlp, grid = c.get_params()
+    grid = rng.uniform(-1, 1, grid.size).astype(np.float32)  # grid features large enough to matter
+    c.set_params(mlp, grid)
+    return c
+
+
+def _oracle(c):
+    mlp, grid = c.get_params()
+    return oracle.OracleCache(c.info(), mlp, grid)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("precision,fwd_tol,grad_tol", [(N.VR_CACHE_FP32, 1e-6, 1e-6), (N.VR_CACHE_BF16, 2e-2, None)])
+def test_forward_and_gradients_match_fp64_oracle(gpu_ctx, precision, fwd_tol, grad_tol):
+    import torch
+    name = "fp32" if precision == N.VR_CACHE_FP32 else "bf16"
+    announce(f"radiance cache {name}: forward + gradients vs FP64 oracle")
+    c = _cache(gpu_ctx, precision)
+    n = 1000
+    pos, d, nr, rough, target = _inputs(n)
+    tp, td, tn, tr, tt = _dev(pos, d, nr, rough, target)
+    out = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    gw = torch.zeros(N.VR_CACHE_MLP_PARAMS, dtype=torch.float32, device="cuda")
+    info = c.info()
+    gg = torch.zeros(info["grid_params"], dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    c.infer(n, tp.data_ptr(), td.data_ptr(), tn.data_ptr(), tr.data_ptr(), out.data_ptr())
+    c.gradients(n, tp.data_ptr(), td.data_ptr(), tn.data_ptr(), tr.data_ptr(), tt.data_ptr(), gw.data_ptr(),
+                gg.data_ptr(), loss.data_ptr())
+    torch.cuda.synchronize()  # a device synchronisation also covers the library's own stream
+    oc = _oracle(c)
+    y = oc.forward(pos, d, nr, rough)
+    ol, ogw, ogg = oc.gradients(pos, d, nr, rough, target)
+    ef = _rel(out.cpu().numpy(), y)
+    el = abs(float(loss.cpu()[0]) - ol) / ol
+    g = gw.cpu().numpy()
+    layer_err = [_rel(g[OFF[k]:OFF[k + 1]], ogw[OFF[k]:OFF[k + 1]]) for k in range(5)]
+    eg = _rel(gg.cpu().numpy(), ogg)
+    print(f"[cache {name}] forward {ef:.2e}, loss {el:.2e}, layers {[f'{e:.2e}' for e in layer_err]}, grid {eg:.2e}",
+          flush=True)
+    assert ef <= fwd_tol
+    assert el <= fwd_tol
+    if grad_tol is not None:
+        assert max(layer_err) <= grad_tol, layer_err
+        assert eg <= grad_tol
+    else:
+        # the same network under the BF16 kernels' rounding model, FP64 accumulation
+        y16 = oc.forward(pos, d, nr, rough, bf16=True)
+        _, bw, bg = oc.gradients(pos, d, nr, rough, target, bf16=True)
+        ef16 = _rel(out.cpu().numpy(), y16)
+        lw16 = [_rel(g[OFF[k]:OFF[k + 1]], bw[OFF[k]:OFF[k + 1]]) for k in range(5)]
+        eg16 = _rel(gg.cpu().numpy(), bg)
+        # conditioning of the grid gradient: sum of |terms| over |sum of terms|
+        _, _, ag = oc.gradients(pos, d, nr, rough, target, abs_mode=True)
+        cond = float(np.linalg.norm(ag) / np.linalg.norm(ogg))
+        print(f"[cache {name}] vs BF16 rounding model: forward {ef16:.2e}, layers {[f'{e:.2e}' for e in lw16]}, "
+              f"grid {eg16:.2e}; grid conditioning {cond:.1f}", flush=True)
+        assert ef16 <= 1e-3
+        assert max(lw16) <= 5e-3 and eg16 <= 5e-3
+        assert max(layer_err) <= 3e-2
+    c.close()
+
+
+def _field(pos, d):
+    return np.stack([1.0 + 0.5 * np.sin(1.5 * pos[:, 0]) * np.cos(pos[:, 2]),
+                     0.8 + 0.4 * d[:, 1] + 0.2 * np.sin(pos[:, 1]),
+                     0.5 + 0.3 * np.cos(2.0 * pos[:, 0] + pos[:, 1])], 1).astype(np.float32)
+
+
+def test_training_fits_a_smooth_field(gpu_ctx):
+    import torch
+    announce("radiance cache bf16: 300 Adam steps on a synthetic field")
+    c = RadianceCache(gpu_ctx, LO, HI, log2_table=14, base_resolution=4, max_resolution=128, seed=5,
+                      lr_grid=1e-2, lr_mlp=1e-2)
+    n = 4096
+    vp, vd, vn, vr, _ = _inputs(n, seed=100)
+    vt = _field(vp, vd)
+    losses = []
+    dv = _dev(vp, vd, vn, vr, vt)
+    vloss = torch.zeros(1, dtype=torch.float32, device="cuda")
+
+    def val_loss():
+        c.gradients(n, *[t.data_ptr() for t in dv], None, None, vloss.data_ptr())
+        torch.cuda.synchronize()
+        return float(vloss.cpu()[0])
+
+    l0 = val_loss()
+    for step in range(300):
+        p, d, nr, r, _ = _inputs(n, seed=1000 + step)
+        t = _field(p, d)
+        tp, td, tn, tr, tt = _dev(p, d, nr, r, t)
+        c.train(n, tp.data_ptr(), td.data_ptr(), tn.data_ptr(), tr.data_ptr(), tt.data_ptr())
+        if step % 100 == 99:
+            losses.append(c.stats()["loss"])
+    l1 = val_loss()
+    print(f"[cache train] validation loss {l0:.4f} -> {l1:.5f}; training losses {losses}", flush=True)
+    assert c.info()["steps"] == 300
+    assert l1 * 10 <= l0
+    c.close()
+
+
+def test_cached_render_trains_on_its_own_paths(gpu_ctx):
+    import torch
+    from voidray_amd import workloads as W
+    from voidray_amd.device import make_render_desc, scene_bounds
+    from voidray_amd.scene import RenderSettings
+    announce("cached render 96x54, 6 frames x 2 spp")
+    scene = W.terrain_spheres(grid=24, n_spheres=6, sphere_res=(16, 8), tex_size=64, env_size=(64, 32),
+                              sun_peak=200.0)
+    accel = scene.build_acceleration(gpu_ctx)
+    lo, hi = scene_bounds(scene)
+    c = RadianceCache(gpu_ctx, lo, hi, log2_table=14, base_resolution=8, max_resolution=256)
+    w, h, spp, frames = 96, 54, 2, 6
+    img = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st = RenderSettings(total_samples=spp * frames, max_bounces=12, firefly_clamp=float("inf"))
+    flags = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+    stats = []
+    for f in range(frames):
+        desc = make_render_desc(w, h, spp, spp * frames, st, sample_offset=f * spp, seed=11, flags=flags, rr_depth=4)
+        c.render_frame(accel, img.data_ptr(), desc, train_paths=2048, train_vertices=4, cache_depth=2,
+                       train_steps=2, frame=f)
+        stats.append(c.stats())
+    torch.cuda.synchronize()
+    im = img.cpu().numpy()
+    print(f"[cached render] stats {stats[-1]}, mean {im[..., :3].mean():.4f}", flush=True)
+    assert np.isfinite(im).all()
+    assert np.all(im[..., 3] == frames)
+    assert im[..., :3].mean() > 0
+    for s in stats:
+        assert 0 < s["train_samples"] <= 2048 * 4
+        assert 0 < s["queries"] <= w * h * spp
+        assert np.isfinite(s["loss"])
+    assert c.info()["steps"] == 2 * frames
+    c.close()

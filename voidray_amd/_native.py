@@ -107,6 +107,35 @@ class vr_tuning(C.Structure):
     _fields_ = [("chunk_paths", C.c_uint32), ("block", C.c_uint32)]
 
 
+VR_CACHE_BF16 = 0
+VR_CACHE_FP32 = 1
+VR_CACHE_LEVELS = 16
+VR_CACHE_MLP_PARAMS = 21504
+
+
+class vr_cache_desc(C.Structure):
+    _fields_ = [("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
+                ("precision", C.c_uint32), ("bbox_lo", C.c_float * 3), ("bbox_hi", C.c_float * 3),
+                ("lr_grid", C.c_float), ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("seed", C.c_uint64)]
+
+
+class vr_cache_info(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("table_size", C.c_uint32), ("precision", C.c_uint32), ("pad", C.c_uint32),
+                ("resolution", C.c_uint32 * 16), ("origin", C.c_float * 3), ("inv_extent", C.c_float),
+                ("grid_params", C.c_uint64), ("steps", C.c_uint64)]
+
+
+class vr_cache_frame(C.Structure):
+    _fields_ = [("train_paths", C.c_uint32), ("train_vertices", C.c_uint32), ("cache_depth", C.c_uint32),
+                ("train_steps", C.c_uint32), ("frame", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class vr_cache_stats(C.Structure):
+    _fields_ = [("loss", C.c_double), ("train_samples", C.c_uint64), ("queries", C.c_uint64),
+                ("train_ms", C.c_double), ("render_ms", C.c_double)]
+
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -131,6 +160,16 @@ SIGNATURES = {
     "vr_batch_last_stats": (C.c_int, [_P, C.POINTER(vr_render_stats)]),
     "vr_batch_autotune": (C.c_int, [_P, C.POINTER(vr_request), C.c_uint32, C.c_uint32, C.c_uint32, _P,
                                     C.POINTER(vr_tuning), C.POINTER(C.c_float)]),
+    "vr_cache_create": (C.c_int, [_P, C.POINTER(vr_cache_desc), C.POINTER(_P)]),
+    "vr_cache_destroy": (C.c_int, [_P]),
+    "vr_cache_get_info": (C.c_int, [_P, C.POINTER(vr_cache_info)]),
+    "vr_cache_get_params": (C.c_int, [_P, _P, _P]),
+    "vr_cache_set_params": (C.c_int, [_P, _P, _P]),
+    "vr_cache_infer_async": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P]),
+    "vr_cache_gradients_async": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_cache_train_async": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P]),
+    "vr_render_cached_async": (C.c_int, [_P, _P, C.POINTER(vr_render_desc), C.POINTER(vr_cache_frame), _P, _P]),
+    "vr_cache_last_stats": (C.c_int, [_P, C.POINTER(vr_cache_stats)]),
     "vr_tonemap": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P]),
     "vr_tonemap_device_async": (C.c_int, [_P, C.POINTER(vr_post_desc), _P, _P, _P]),
 }

@@ -231,6 +231,16 @@ int vr_render_batch_async(vr_ctx* ctx, const vr_request* requests, uint32_t coun
     });
 }
 
+int vr_render_cached_async(vr_scene* s, vr_cache* cache, const vr_render_desc* desc, const vr_cache_frame* frame,
+                           float* d_rgba, void* stream) {
+    return guarded([&] {
+        if (!s || !cache || !desc || !frame || !d_rgba) return fail(VR_ERR_INVALID, "null argument");
+        VR_HIP_CHECK(hipSetDevice(s->ctx->device));
+        hipStream_t st = stream ? (hipStream_t)stream : s->ctx->stream;
+        return render_cached(s, cache, desc, frame, d_rgba, st);
+    });
+}
+
 int vr_batch_relaunch_async(vr_ctx* ctx, void* stream) {
     return guarded([&] {
         if (!ctx) return fail(VR_ERR_INVALID, "null argument");

@@ -4,9 +4,20 @@
 namespace vr {
 // One request: a batch of one.
 int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, hipStream_t st);
+// Radiance-cache role of one request of a batch.
+struct CacheReq {
+    uint32_t cmode;           // 0 none, 1 training paths, 2 cache queries
+    uint32_t cache_depth;     // query vertex
+    uint32_t train_vertices;  // recorded vertices per training path
+    uint32_t train_paths;     // training paths (replace the image's pixels)
+    uint32_t train_stride, train_offset;
+};
 // Plans, uploads and enqueues a batch (optionally captured into a graph and launched).
 int render_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning* tuning, uint32_t flags,
-                 hipStream_t st);
+                 hipStream_t st, const CacheReq* creq = nullptr, vr_cache* cache = nullptr);
+// One frame of the cached renderer.
+int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr_cache_frame* f, float* d_rgba,
+                  hipStream_t st);
 // Launches the last captured batch again.
 int relaunch_batch(vr_ctx* ctx, hipStream_t st);
 // Counters of the last render or batch on the context (waits for its read-back).

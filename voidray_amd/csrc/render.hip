@@ -31,6 +31,7 @@
 #include <map>
 #include <tuple>
 
+#include "nrc.h"
 #include "render.h"
 #include "shade.h"
 
@@ -50,6 +51,9 @@ enum : int {
     kCntWords
 };
 
+// radiance-cache role of a request
+enum : uint32_t { kCacheNone = 0, kCacheTrain = 1, kCacheQuery = 2 };
+
 struct ReqDev {
     float* rgba;
     uint64_t seed;
@@ -57,6 +61,11 @@ struct ReqDev {
     int32_t mode;
     float clamp;
     uint32_t ext;
+    uint32_t npix;            // paths / spp: W*H, or the number of training paths
+    uint32_t cmode;           // kCache*
+    uint32_t cache_depth;     // query vertex
+    uint32_t train_vertices;  // recorded vertices per training path
+    uint32_t train_stride, train_offset;  // training path k traces pixel (k*stride + offset) mod W*H
 };
 
 struct SegDev {
@@ -85,6 +94,7 @@ struct RenderArgs {
     uint32_t* queue[2];
     uint32_t* sh_queue;
     uint32_t* cnt;
+    NrcSamples tr, qr;  // radiance cache: training samples, queries
 };
 
 __device__ __forceinline__ vf3 ld3(const float* b, uint32_t p) {
@@ -120,6 +130,26 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t value, uint32_t*
     if (want) queue[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Slot index for each lane that wants one (one atomic per wave); UINT32_MAX otherwise.
+__device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
+    uint64_t mask = __ballot(want);
+    if (mask == 0) return 0xFFFFFFFFu;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return want ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
+}
+
+// Roughness input of the radiance cache: 1 for diffuse and emissive
+// surfaces, the fuzz of Metal, 0 for Dielectric.
+__device__ __forceinline__ float cache_roughness(const GpuMaterial& m) {
+    if (m.kind == VR_MAT_METAL) return fminf(fmaxf(m.param, 0.f), 1.f);
+    if (m.kind == VR_MAT_DIELECTRIC) return 0.f;
+    return 1.f;
+}
+
 __global__ void k_render_zero(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -135,6 +165,8 @@ __global__ void k_raygen(RenderArgs a) {
     uint32_t local = p - sg.path_base;
     uint32_t i = sg.pix0 + local / r.spp;
     uint32_t s = r.sample_offset + local % r.spp;
+    if (r.cmode == kCacheTrain)  // training paths spread over the image by a stride coprime to W*H
+        i = (uint32_t)(((uint64_t)i * r.train_stride + r.train_offset) % ((uint64_t)r.W * r.H));
     a.path_req[p] = sg.req;
     a.path_pix[p] = i;
     a.path_smp[p] = s;
@@ -196,6 +228,9 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
         bool act = k < n;
         uint32_t p = 0;
         bool cont = false, shadow = false, miss = false;
+        uint32_t rec = 0;  // radiance cache: 1 = training sample, 2 = query
+        vf3 rec_pos{}, rec_dir{}, rec_n{}, rec_A{}, rec_L{};
+        float rec_rough = 0.f;
         uint32_t capped = 0;
         if (act) {
             p = qin ? qin[k] : k;
@@ -249,7 +284,21 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                     }
                 } else {
                     vf3 A = ld3(a.A, p), L = ld3(a.L, p);
-                    if (m.kind == VR_MAT_EMISSION) {
+                    const bool query = r.cmode == kCacheQuery && depth == r.cache_depth;
+                    if (query || (r.cmode == kCacheTrain && depth < r.train_vertices)) {
+                        // the vertex, its outgoing direction and the path state before its contributions
+                        vf3 ns = m.normal_tex >= 0 ? tex_sample(sv, m.normal_tex, hit.uv.x, hit.uv.y) : hit.normal;
+                        rec = query ? 2u : 1u;
+                        rec_pos = hit.point;
+                        rec_dir = -d;
+                        rec_n = normalize3(ns);
+                        rec_rough = cache_roughness(m);
+                        rec_A = A;
+                        rec_L = L;
+                    }
+                    if (query) {
+                        // the path ends: the cache supplies the radiance leaving this vertex
+                    } else if (m.kind == VR_MAT_EMISSION) {
                         st3(a.L, p, L + mulv(A, mk3(m.color[0], m.color[1], m.color[2])));
                     } else {
                         if (m.kind == VR_MAT_LAMBERTIAN && (r.flags & VR_RENDER_ENV_IS) &&
@@ -297,6 +346,23 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
             }
             a.rng_n[p] = rng.n;
         }
+        {
+            const uint32_t slot_t = wave_alloc(rec == 1u, a.tr.count);
+            const uint32_t slot_q = wave_alloc(rec == 2u, a.qr.count);
+            if (rec) {
+                const bool t = rec == 1u;
+                const uint32_t slot = t ? slot_t : slot_q;
+                if (slot < (t ? a.tr.cap : a.qr.cap)) {
+                    st3(t ? a.tr.pos : a.qr.pos, slot, rec_pos);
+                    st3(t ? a.tr.dir : a.qr.dir, slot, rec_dir);
+                    st3(t ? a.tr.nrm : a.qr.nrm, slot, rec_n);
+                    (t ? a.tr.rough : a.qr.rough)[slot] = rec_rough;
+                    (t ? a.tr.path : a.qr.path)[slot] = p;
+                    st3(t ? a.tr.A : a.qr.A, slot, rec_A);
+                    st3(t ? a.tr.L : a.qr.L, slot, rec_L);
+                }
+            }
+        }
         wave_append(cont, p, qout, a.cnt + kCntActive + depth + 1);
         wave_append(shadow, p, a.sh_queue, a.cnt + kCntShadow + depth);
         uint64_t mm = __ballot(miss);
@@ -313,6 +379,33 @@ __global__ void k_shadow_resolve(RenderArgs a, uint32_t depth) {
     }
 }
 
+// Cache queries of the chunk: L[path] += A * max(cache output, 0).
+__global__ void k_cache_apply(RenderArgs a) {
+    const uint32_t n = min(*a.qr.count, a.qr.cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t p = a.qr.path[k];
+        vf3 y = ld3(a.qr.target, k);
+        y = mk3(fmaxf(y.x, 0.f), fmaxf(y.y, 0.f), fmaxf(y.z, 0.f));
+        st3(a.L, p, ld3(a.L, p) + mulv(ld3(a.qr.A, k), y));
+    }
+}
+
+// Training targets: the radiance a path gathered from vertex k on, divided by
+// the throughput that arrived at vertex k, (L_final - L_k) / A_k per channel.
+__global__ void k_cache_targets(RenderArgs a) {
+    const uint32_t n = min(*a.tr.count, a.tr.cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t p = a.tr.path[k];
+        vf3 lf = ld3(a.L, p), lk = ld3(a.tr.L, k), ak = ld3(a.tr.A, k);
+        float t[3];
+        for (int c = 0; c < 3; ++c) {
+            float v = comp(ak, c) > 1e-20f ? (comp(lf, c) - comp(lk, c)) / comp(ak, c) : 0.f;
+            t[c] = (v == v && fabsf(v) < 3.0e38f) ? fmaxf(v, 0.f) : 0.f;
+        }
+        st3(a.tr.target, k, mk3(t[0], t[1], t[2]));
+    }
+}
+
 // Per pixel: sum of the samples in index order, scaled by 1/total_samples
 // and added into the request's target (iterative.rs:45-51).
 __global__ void k_accumulate(RenderArgs a) {
@@ -321,6 +414,7 @@ __global__ void k_accumulate(RenderArgs a) {
     uint32_t k = seg_find(a.segs, a.nseg, q, [](const SegDev& s) { return s.pix_base; });
     const SegDev sg = a.segs[k];
     const ReqDev& r = a.reqs[sg.req];
+    if (r.cmode == kCacheTrain) return;  // training paths feed the cache, not the image
     uint32_t local = q - sg.pix_base;
     uint32_t base = sg.path_base + local * r.spp;
     vf3 color = mk3(0.f, 0.f, 0.f);
@@ -429,6 +523,7 @@ struct Group {
     vr_scene* scene = nullptr;
     uint32_t engine = 0;
     bool any_ext = false, any_env_is = false;
+    bool any_train = false, any_query = false;  // radiance-cache roles
     uint32_t att_levels = 0;  // max bounces among reference-estimator requests
     uint32_t max_depth = 0;
     uint64_t paths = 0;
@@ -446,6 +541,7 @@ struct RenderState {
     std::vector<Group> groups;
     uint32_t cap = 0;
     vr_tuning tune{};
+    vr_cache* cache = nullptr;  // radiance cache of the batch being planned (cached frames only)
     bool stamps = false;
     // device table: [reqs][segs][done ids][stamps]; staged through pinned host memory
     void* table = nullptr;
@@ -490,7 +586,7 @@ static void drop_graph(RenderState& R) {
 // Plans the chunks of a batch and allocates everything it needs (never inside
 // a graph capture).
 static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning& tune, uint32_t flags,
-                      hipStream_t st) {
+                      hipStream_t st, const CacheReq* creq) {
     RenderState& R = *ctx->render;
     uint32_t chunk_paths = tune.chunk_paths ? tune.chunk_paths : kDefaultChunkPaths;
     if (chunk_paths > kMaxChunkPaths) return fail(VR_ERR_INVALID, "chunk_paths above 2^26");
@@ -532,12 +628,21 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         r.mode = d.render_mode;
         r.clamp = d.firefly_clamp;
         r.ext = is_ext(d) ? 1u : 0u;
+        r.npix = d.width * d.height;
+        if (creq) {
+            r.cmode = creq[i].cmode;
+            r.cache_depth = creq[i].cache_depth;
+            r.train_vertices = creq[i].train_vertices;
+            r.train_stride = creq[i].train_stride;
+            r.train_offset = creq[i].train_offset;
+            if (r.cmode == kCacheTrain) r.npix = d.width * d.height ? creq[i].train_paths : 0;
+        }
         R.reqs.push_back(r);
-        uint64_t rp = (uint64_t)d.width * d.height * d.samples;
+        uint64_t rp = (uint64_t)r.npix * d.samples;
         uint32_t size_class = 0;
         if (sjf)
             while (rp >> (2 * (size_class + 1))) ++size_class;
-        auto key = std::make_tuple(it.scene, d.engine, size_class);
+        auto key = std::make_tuple(it.scene, d.engine, size_class + (r.cmode << 16));
         auto f = gidx.find(key);
         if (f == gidx.end()) {
             Group g;
@@ -551,13 +656,17 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         g.any_ext |= r.ext != 0;
         g.any_env_is |= r.ext && (d.flags & VR_RENDER_ENV_IS) && it.scene->view.env_kind == VR_ENV_HDRI;
         if (!r.ext) g.att_levels = std::max(g.att_levels, d.max_bounces);
-        g.max_depth = std::max(g.max_depth, d.max_bounces);
-        g.paths += (uint64_t)d.width * d.height * d.samples;
+        // query paths end at the query vertex
+        uint32_t depth = r.cmode == kCacheQuery ? std::min(d.max_bounces, r.cache_depth + 1) : d.max_bounces;
+        g.max_depth = std::max(g.max_depth, depth);
+        g.paths += rp;
+        g.any_train |= r.cmode == kCacheTrain;
+        g.any_query |= r.cmode == kCacheQuery;
     }
     if (flags & VR_BATCH_SHORTEST_FIRST) {
         std::stable_sort(R.groups.begin(), R.groups.end(),
                          [](const Group& a, const Group& b) { return a.paths < b.paths; });
-        auto paths = [&](uint32_t i) { return (uint64_t)R.reqs[i].W * R.reqs[i].H * R.reqs[i].spp; };
+        auto paths = [&](uint32_t i) { return (uint64_t)R.reqs[i].npix * R.reqs[i].spp; };
         for (Group& g : R.groups)
             std::stable_sort(g.reqs.begin(), g.reqs.end(),
                              [&](uint32_t a, uint32_t b) { return paths(a) < paths(b); });
@@ -573,7 +682,7 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         std::vector<std::pair<uint32_t, uint32_t>> finish;  // (request, chunk index)
         for (uint32_t ri : g.reqs) {
             const ReqDev& r = R.reqs[ri];
-            uint64_t npx = (uint64_t)r.W * r.H;
+            uint64_t npx = r.npix;
             if (r.spp == 0 || npx == 0) {
                 finish.emplace_back(ri, UINT32_MAX);  // nothing to render
                 continue;
@@ -713,6 +822,11 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
         a.queue[0] = carve<uint32_t>(cur, cap);
         a.queue[1] = carve<uint32_t>(cur, cap);
         a.cnt = carve<uint32_t>(cur, kCntWords);
+        if (g.any_train || g.any_query) {
+            if (!R.cache) return fail(VR_ERR_INVALID, "radiance-cache request without a cache");
+            if (g.any_train) a.tr = R.cache->tr;
+            if (g.any_query) a.qr = R.cache->qr;
+        }
         for (const Chunk& c : g.chunks) {
             if (c.npaths) {
                 a.segs = R.d_segs + c.seg_off;
@@ -721,6 +835,8 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                 a.npaths = c.npaths;
                 hipLaunchKernelGGL(k_render_zero, dim3((kCntWords + 255) / 256), dim3(256), 0, st, a.cnt,
                                    (uint32_t)kCntWords);
+                if (g.any_train) hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.tr.count, 1u);
+                if (g.any_query) hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.qr.count, 1u);
                 hipLaunchKernelGGL(k_raygen, dim3((c.npaths + sb - 1) / sb), dim3(sb), 0, st, a);
                 for (uint32_t depth = 0; depth < g.max_depth; ++depth) {
                     TraceJob j;
@@ -758,6 +874,13 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                         hipLaunchKernelGGL(k_shadow_resolve, dim3(kShadeGrid), dim3(sb), 0, st, a, depth);
                     }
                 }
+                if (g.any_query) {
+                    int rc = nrc_infer(R.cache, a.qr.cap, a.qr.count, a.qr.pos, a.qr.dir, a.qr.nrm, a.qr.rough,
+                                       a.qr.target, st);
+                    if (rc) return rc;
+                    hipLaunchKernelGGL(k_cache_apply, dim3(kShadeGrid), dim3(sb), 0, st, a);
+                }
+                if (g.any_train) hipLaunchKernelGGL(k_cache_targets, dim3(kShadeGrid), dim3(sb), 0, st, a);
                 hipLaunchKernelGGL(k_accumulate, dim3((c.npix + sb - 1) / sb), dim3(sb), 0, st, a);
                 hipLaunchKernelGGL(k_stats_chunk, dim3(1), dim3(64), 0, st, (const uint32_t*)a.cnt, R.stats_dev,
                                    c.npaths, g.max_depth);
@@ -784,12 +907,13 @@ static int enqueue_readback(vr_ctx* ctx, hipStream_t st) {
 }
 
 int render_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning* tuning, uint32_t flags,
-                 hipStream_t st) {
+                 hipStream_t st, const CacheReq* creq, vr_cache* cache) {
     int rc = ensure_state(ctx);
     if (rc) return rc;
     RenderState& R = *ctx->render;
     vr_tuning tune = tuning ? *tuning : vr_tuning{};
-    rc = plan_batch(ctx, items, n, tune, flags, st);
+    R.cache = cache;
+    rc = plan_batch(ctx, items, n, tune, flags, st, creq);
     if (rc) return rc;
     R.stamps = (flags & VR_BATCH_TIMESTAMPS) != 0;
     if (!(flags & VR_BATCH_GRAPH)) {
@@ -826,6 +950,72 @@ int relaunch_batch(vr_ctx* ctx, hipStream_t st) {
 int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, hipStream_t st) {
     vr_request r{s, *desc, d_rgba};
     return render_batch(s->ctx, &r, 1, nullptr, 0, st);
+}
+
+static uint32_t gcd32(uint32_t a, uint32_t b) {
+    while (b) {
+        uint32_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// One frame of the cached renderer: training paths -> samples -> Adam steps,
+// then render paths that stop at the query vertex and take the cache's answer.
+int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr_cache_frame* f, float* d_rgba,
+                  hipStream_t st) {
+    vr_ctx* ctx = s->ctx;
+    if (c->ctx != ctx) return fail(VR_ERR_INVALID, "cache and scene belong to different contexts");
+    if (!is_ext(*desc))
+        return fail(VR_ERR_INVALID, "the cached renderer runs the extended estimator (flags ENV_IS and/or RUSSIAN_ROULETTE)");
+    if (f->train_vertices == 0 || f->train_vertices > kNrcMaxTrainVertices)
+        return fail(VR_ERR_INVALID, "train_vertices must be in [1, 8]");
+    if (f->train_paths > kMaxChunkPaths) return fail(VR_ERR_INVALID, "train_paths above 2^26");
+    if (f->cache_depth >= desc->max_bounces) return fail(VR_ERR_INVALID, "cache_depth must be below max_bounces");
+    const uint64_t npx = (uint64_t)desc->width * desc->height;
+    if (npx == 0) return 0;
+    if (npx >= (1ull << 32)) return fail(VR_ERR_INVALID, "image too large");
+    const uint64_t train_cap = (uint64_t)f->train_paths * f->train_vertices;
+    if (train_cap > (1ull << 28)) return fail(VR_ERR_INVALID, "too many training samples per frame");
+    const uint32_t qcap = (uint32_t)std::min<uint64_t>(npx * std::max(desc->samples, 1u), kDefaultChunkPaths);
+    int rc = ensure_state(ctx);
+    if (!rc) rc = nrc_ensure_frame_buffers(c, (uint32_t)train_cap, qcap);
+    if (rc) return rc;
+    VR_HIP_CHECK(hipEventRecord(c->ev[0], st));
+    if (f->train_paths && f->train_steps) {
+        vr_request tr{s, *desc, d_rgba};
+        tr.desc.samples = 1;
+        tr.desc.total_samples = 1;
+        tr.desc.sample_offset = f->frame;
+        tr.desc.request = desc->request ^ 0x80000000u;  // streams apart from the image's paths
+        CacheReq cq{};
+        cq.cmode = kCacheTrain;
+        cq.train_vertices = f->train_vertices;
+        cq.train_paths = f->train_paths;
+        uint32_t stride = (uint32_t)(2654435761ull % npx);
+        if (stride == 0) stride = 1;
+        while (gcd32(stride, (uint32_t)npx) != 1) ++stride;
+        cq.train_stride = stride;
+        cq.train_offset = (uint32_t)(((uint64_t)f->frame * 2246822519ull + 12345ull) % npx);
+        vr_tuning t{std::max<uint32_t>(kDefaultChunkPaths, (f->train_paths + 63) & ~63u), 0};
+        rc = render_batch(ctx, &tr, 1, &t, 0, st, &cq, c);
+        if (rc) return rc;
+        for (uint32_t k = 0; k < f->train_steps && !rc; ++k)
+            rc = nrc_train_step(c, c->tr.cap, c->tr.count, c->tr.pos, c->tr.dir, c->tr.nrm, c->tr.rough, c->tr.target,
+                                true, st);
+        if (rc) return rc;
+    }
+    VR_HIP_CHECK(hipEventRecord(c->ev[1], st));
+    CacheReq cr{};
+    cr.cmode = kCacheQuery;
+    cr.cache_depth = f->cache_depth;
+    vr_request rq{s, *desc, d_rgba};
+    rc = render_batch(ctx, &rq, 1, nullptr, 0, st, &cr, c);
+    if (rc) return rc;
+    VR_HIP_CHECK(hipEventRecord(c->ev[2], st));
+    c->frame_timed = true;
+    return nrc_frame_readback(c, st);
 }
 
 // Statistics of the last render or batch on this context (waits for it).

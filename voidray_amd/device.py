@@ -166,6 +166,91 @@ class BatchRenderer:
         return (best.chunk_paths, best.block), ms.value
 
 
+class RadianceCache:
+    """Learned radiance cache (vr_cache_*): multiresolution hash grid + MLP,
+    BF16 on MFMA (precision=VR_CACHE_FP32: the all-FP32 debug path), trained
+    online on path suffixes the tracer produces.  No counterpart in the
+    reference, whose integrator traces every path to the end
+    (core/tracer.rs:9-56)."""
+
+    def __init__(self, ctx, bbox_lo, bbox_hi, log2_table=19, base_resolution=16, max_resolution=2048,
+                 precision=N.VR_CACHE_BF16, lr_grid=1e-2, lr_mlp=1e-3, beta1=0.9, beta2=0.99, eps=1e-15, seed=0):
+        d = N.vr_cache_desc()
+        d.log2_table, d.base_resolution, d.max_resolution, d.precision = log2_table, base_resolution, \
+            max_resolution, precision
+        for a in range(3):
+            d.bbox_lo[a], d.bbox_hi[a] = float(bbox_lo[a]), float(bbox_hi[a])
+        d.lr_grid, d.lr_mlp, d.beta1, d.beta2, d.eps, d.seed = lr_grid, lr_mlp, beta1, beta2, eps, seed
+        h = C.c_void_p()
+        N.check(N.lib().vr_cache_create(ctx.handle, C.byref(d), C.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+
+    def info(self):
+        i = N.vr_cache_info()
+        N.check(N.lib().vr_cache_get_info(self.handle, C.byref(i)))
+        return {"levels": i.levels, "table_size": i.table_size, "log2_table": int(i.table_size).bit_length() - 1,
+                "precision": i.precision, "resolution": list(i.resolution), "origin": list(i.origin),
+                "inv_extent": i.inv_extent, "grid_params": i.grid_params, "steps": i.steps}
+
+    def get_params(self):
+        info = self.info()
+        mlp = np.zeros(N.VR_CACHE_MLP_PARAMS, np.float32)
+        grid = np.zeros(info["grid_params"], np.float32)
+        N.check(N.lib().vr_cache_get_params(self.handle, mlp.ctypes.data, grid.ctypes.data))
+        return mlp, grid
+
+    def set_params(self, mlp=None, grid=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        N.check(N.lib().vr_cache_set_params(self.handle, None if m is None else m.ctypes.data,
+                                            None if g is None else g.ctypes.data))
+
+    def infer(self, n, d_pos, d_dir, d_nrm, d_rough, d_out, stream=None):
+        N.check(N.lib().vr_cache_infer_async(self.handle, n, d_pos, d_dir, d_nrm, d_rough, d_out, stream))
+
+    def gradients(self, n, d_pos, d_dir, d_nrm, d_rough, d_target, d_mlp_grad=None, d_grid_grad=None,
+                  d_loss=None, stream=None):
+        N.check(N.lib().vr_cache_gradients_async(self.handle, n, d_pos, d_dir, d_nrm, d_rough, d_target,
+                                                 d_mlp_grad, d_grid_grad, d_loss, stream))
+
+    def train(self, n, d_pos, d_dir, d_nrm, d_rough, d_target, stream=None):
+        N.check(N.lib().vr_cache_train_async(self.handle, n, d_pos, d_dir, d_nrm, d_rough, d_target, stream))
+
+    def render_frame(self, accel, d_rgba, desc, train_paths=1 << 14, train_vertices=4, cache_depth=2,
+                     train_steps=1, frame=0, stream=None):
+        f = N.vr_cache_frame(train_paths, train_vertices, cache_depth, train_steps, frame, 0)
+        N.check(N.lib().vr_render_cached_async(accel.handle, self.handle, C.byref(desc), C.byref(f), d_rgba,
+                                               stream))
+
+    def stats(self):
+        s = N.vr_cache_stats()
+        N.check(N.lib().vr_cache_last_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in N.vr_cache_stats._fields_}
+
+    def close(self):
+        if self.handle:
+            N.lib().vr_cache_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_bounds(scene):
+    """Axis-aligned bounds of every mesh vertex of a Scene (for the cache's position normalisation)."""
+    lo = np.full(3, np.inf, np.float64)
+    hi = np.full(3, -np.inf, np.float64)
+    for m in scene.meshes:
+        if len(m.positions):
+            lo = np.minimum(lo, m.positions.min(0))
+            hi = np.maximum(hi, m.positions.max(0))
+    return lo.astype(np.float32), hi.astype(np.float32)
+
+
 def make_render_desc(width, height, samples, total_samples, settings=None, sample_offset=0, seed=0, request=0,
                      flags=N.VR_RENDER_ALPHA_PER_CALL, rr_depth=0, engine=N.VR_ENGINE_WIDE):
     from .scene import RenderSettings
