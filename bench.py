@@ -57,6 +57,143 @@ def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
                       f"1,048,576-triangle scene; reference median-split tree built in {build_s:.1f} s (untimed)"}
 
 
+_TERRAIN = {}
+
+
+def _terrain(ctx):
+    """The 1080p path-integration scene (2.9 M triangles), built once per run."""
+    if "scene" not in _TERRAIN:
+        from voidray_amd import workloads as W
+        s = W.terrain_spheres()
+        _TERRAIN["scene"], _TERRAIN["accel"] = s, s.build_acceleration(ctx)
+    return _TERRAIN["scene"], _TERRAIN["accel"]
+
+
+def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096):
+    """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
+    + MIS) at width x height, judged against the plain tracer at ref_spp by
+    relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
+    Plain: progressive 4-spp passes, image snapshot after each.  Cached: 8
+    warm-up frames of 1 spp (rendered into a discarded image, counted in its
+    time), then 32 frames of 4 spp.  In each frame ~2^16 randomly chosen paths
+    run to the end as training paths and record up to 4 vertices with their
+    suffix radiance; the others stop at vertex `cache_depth` and add the
+    cache's answer; 8 Adam steps on disjoint slices of the samples follow.  The same
+    run with training off (untrained cache: truncation alone) is the control.
+    Device time per pass from HIP events; snapshots are excluded."""
+    import numpy as np
+
+    from voidray_amd import _native as N
+    from voidray_amd.device import RadianceCache, make_render_desc, scene_bounds
+    from voidray_amd.scene import RenderSettings
+    sp = stream.cuda_stream
+    scene, accel = _terrain(ctx)
+    w, h = width, height
+    ext = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+    st = RenderSettings(total_samples=1, max_bounces=16, firefly_clamp=float("inf"))
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        return e
+
+    def relmse(x, r):
+        x = x[..., :3].astype(np.float64)
+        return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+    img = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    e0 = ev()
+    for k in range(0, ref_spp, 64):
+        accel.render_device(img.data_ptr(), make_render_desc(w, h, 64, ref_spp, st, sample_offset=k, seed=4242,
+                                                             flags=ext, rr_depth=4), stream=sp)
+    e1 = ev()
+    torch.cuda.synchronize()
+    ref_ms = e0.elapsed_time(e1)
+    ref = img[..., :3].double().cpu().numpy()
+
+    def curve(passes, run_pass, total):
+        """Runs `passes` passes with an image snapshot after each; returns
+        [(cumulative device ms, spp so far, relMSE of the snapshot rescaled to its spp)]."""
+        img.zero_()
+        torch.cuda.synchronize()
+        spans, snaps = [], []
+        done = 0
+        for k in range(passes):
+            a = ev()
+            done += run_pass(k)
+            b = ev()
+            spans.append((a, b))
+            snaps.append((done, img.clone() if done else None))
+        torch.cuda.synchronize()
+        cum = np.cumsum([a.elapsed_time(b) for a, b in spans])
+        return [(float(cum[k]), d, relmse(s.cpu().numpy() * (total / d), ref))
+                for k, (d, s) in enumerate(snaps) if d]
+
+    spf = 4
+
+    def plain_pass(k):
+        accel.render_device(img.data_ptr(), make_render_desc(w, h, spf, 32 * spf, st, sample_offset=k * spf, seed=7,
+                                                             flags=ext, rr_depth=4), stream=sp)
+        return spf
+
+    plain_pts = curve(32, plain_pass, 32 * spf)
+    plain_img = img[..., :3].double().cpu().numpy()
+    out = {"scene": f"terrain {w}x{h}, {scene.num_triangles} triangles", "reference_spp": ref_spp,
+           "reference_ms": round(ref_ms, 1),
+           "plain": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5)} for t, s, e in plain_pts[::4] + plain_pts[-1:]]}
+    lo, hi = scene_bounds(scene)
+    warm, frames = 8, 32
+    scratch = torch.zeros_like(img)
+
+    def box8(x):
+        hh, ww = x.shape[0] // 8 * 8, x.shape[1] // 8 * 8
+        return x[:hh, :ww, :3].reshape(hh // 8, 8, ww // 8, 8, 3).mean((1, 3))
+
+    ref8 = box8(ref)
+    for depth in (2, 1):
+        for trained in (True, False):
+            cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=2048)
+
+            def cached_pass(k, depth=depth, cache=cache, trained=trained):
+                # warm-up frames (1 spp) train the cache into a discarded image
+                target, off, n = (scratch, 0, 0) if k < warm else (img, (k - warm) * spf, spf)
+                cache.render_frame(accel, target.data_ptr(), make_render_desc(w, h, spf if n else 1, frames * spf, st,
+                                                                              sample_offset=off, seed=7, flags=ext,
+                                                                              rr_depth=4),
+                                   train_paths=1 << 16, train_vertices=4, cache_depth=depth,
+                                   train_steps=8 if trained else 0, frame=k, stream=sp)
+                return n
+
+            pts = curve(warm + frames, cached_pass, frames * spf)
+            stats = cache.stats()
+            cache.close()
+            final = img[..., :3].double().cpu().numpy()
+            # plain relMSE at the cached run's final time: log-log interpolation of the plain curve
+            pt = np.log([p[0] for p in plain_pts])
+            pe = np.log([p[2] for p in plain_pts])
+            t_end, _, e_end = pts[-1]
+            plain_at = float(np.exp(np.interp(np.log(t_end), pt, pe, left=np.nan, right=np.nan)))
+            if not np.isfinite(plain_at):  # beyond the plain curve: 1/t extrapolation from its last point
+                plain_at = plain_pts[-1][2] * plain_pts[-1][0] / t_end
+            # time the cached run needs to reach the plain tracer's final relMSE
+            target_t, target_e = plain_pts[-1][0], plain_pts[-1][2]
+            reach = next((t for t, _, e in pts if e <= target_e), None)
+            out[f"cache_depth_{depth}_{'trained' if trained else 'untrained'}"] = {
+                "points": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5)} for t, s, e in pts[::4] + pts[-1:]],
+                "final": {"ms": round(t_end, 2), "relmse": round(e_end, 5),
+                          "mean_over_reference": round(float(final.mean() / ref.mean()), 4),
+                          "relmse_8x8": round(relmse(box8(final), ref8), 6)},
+                "train_samples_per_frame": stats["train_samples"], "queries_per_frame": stats["queries"],
+                "train_ms_per_frame": round(stats["train_ms"], 3), "render_ms_per_frame": round(stats["render_ms"], 3),
+                "equal_time_relmse_ratio_plain_over_cache": round(plain_at / e_end, 3),
+                "time_to_reach_plain_final_relmse_ratio": (round(reach / target_t, 3) if reach else None)}
+    out["plain_final"] = {"ms": round(plain_pts[-1][0], 2), "relmse": round(plain_pts[-1][2], 5),
+                          "mean_over_reference": round(float(plain_img.mean() / ref.mean()), 4),
+                          "relmse_8x8": round(relmse(box8(plain_img), ref8), 6)}
+    return out
+
+
 def _render_legs(ctx, cpu=True, c3_spp=64):
     """Path-integration legs reported beside the headline metric:
     * cornell 512x512, 16 spp, depth 8, reference estimator (firefly clamp 3);
@@ -81,8 +218,7 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
                                 "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2),
                                 "segments": st["segments"], "exact_engine_rays": st["fallback_rays"]}
     t0 = time.time()
-    c3 = W.terrain_spheres()
-    a3 = c3.build_acceleration(ctx)
+    c3, a3 = _terrain(ctx)
     build3 = time.time() - t0
     s3 = RenderSettings(total_samples=c3_spp, max_bounces=16, firefly_clamp=float("inf"))
     ext = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
@@ -208,6 +344,8 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-integration legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--only-serving", action="store_true", help="run the batched-serving leg alone (prints it)")
+    ap.add_argument("--no-cache", action="store_true", help="skip the learned-radiance-cache leg")
+    ap.add_argument("--only-cache", action="store_true", help="run the learned-radiance-cache leg alone (prints it)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 closest-hit queries of the workload, then a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
@@ -225,6 +363,11 @@ def main():
 
     dev = torch.device("cuda:0")
     ctx = Context(0)
+    if args.only_cache:
+        st = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(st)
+        print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st)}), flush=True)
+        return
     if args.only_serving:
         st = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(st)
@@ -303,6 +446,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     serving = None if args.no_serving else _serving_leg(ctx, torch, stream)
+    cache = None if args.no_cache else _cache_leg(ctx, torch, stream)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),
@@ -337,6 +481,7 @@ def main():
             "gpu_over_cpu": (round(mrays / cpu["value"], 1) if cpu else None),
             "render": renders,
             "serving": serving,
+            "radiance_cache": cache,
         },
     }
     print(json.dumps(out), flush=True)

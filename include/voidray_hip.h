@@ -321,11 +321,11 @@ typedef struct vr_cache_info {
 } vr_cache_info;
 
 typedef struct vr_cache_frame {
-    uint32_t train_paths;     /* training paths traced per frame (their vertices become samples) */
+    uint32_t train_paths;     /* expected training paths per frame, drawn at random among its paths */
     uint32_t train_vertices;  /* vertices per training path recorded as samples (<= 8) */
-    uint32_t cache_depth;     /* render paths query the cache at this vertex (2 = after bounce 2) */
-    uint32_t train_steps;     /* Adam steps per frame over the frame's samples */
-    uint32_t frame;           /* frame index: selects the training pixels and random streams */
+    uint32_t cache_depth;     /* other paths query the cache at this vertex (2 = after bounce 2) */
+    uint32_t train_steps;     /* Adam steps per frame, each on one slice of the frame's samples */
+    uint32_t frame;           /* frame index: varies the choice of training paths */
     uint32_t pad;
 } vr_cache_frame;
 
@@ -359,12 +359,13 @@ int vr_cache_gradients_async(vr_cache* cache, uint32_t n, const float* d_pos, co
 /* One Adam step on the given samples. */
 int vr_cache_train_async(vr_cache* cache, uint32_t n, const float* d_pos, const float* d_dir,
                          const float* d_nrm, const float* d_rough, const float* d_target, void* stream);
-/* One frame of the cached renderer: trace `train_paths` full paths of the
- * extended estimator (flags ENV_IS/RR), record their first `train_vertices`
- * vertices with path-suffix radiance as samples, take `train_steps` Adam
- * steps, then render desc->samples paths per pixel that stop at vertex
- * `cache_depth` and add throughput x cache(vertex).  Accumulates into d_rgba
- * like vr_render_device_async. */
+/* One frame of the cached renderer (extended estimator: flags ENV_IS/RR):
+ * desc->samples paths per pixel; each stops at vertex `cache_depth` and adds
+ * throughput x cache(vertex), except ~train_paths randomly chosen training
+ * paths that run to the end like the plain tracer's and record their first
+ * `train_vertices` vertices with their path-suffix radiance as samples.  All
+ * paths accumulate into d_rgba like vr_render_device_async; then
+ * `train_steps` Adam steps on the frame's samples. */
 int vr_render_cached_async(vr_scene* scene, vr_cache* cache, const vr_render_desc* desc,
                            const vr_cache_frame* frame, float* d_rgba, void* stream);
 /* Statistics of the last frame (waits for it). */

@@ -1,15 +1,18 @@
 """Learned radiance cache on the GPU against the FP64 oracle (oracle/nrc_oracle.cpp).
 
 Bars (relative error = ||gpu - oracle|| / ||oracle|| over the whole array or layer):
-* FP32 debug path: forward, loss and every gradient <= 1e-6;
-* BF16 / MFMA path: forward and loss <= 2e-2; every gradient entry within
-  2^-5 of the sum of the absolute values of its terms (oracle abs mode):
-  BF16 keeps 8 significant bits of inputs, activations and deltas, and an
-  entry that sums many signed per-sample terms can lose most of them to
-  cancellation, so a norm-wise bar alone would measure conditioning, not the
-  kernels (the norm-wise errors are printed);
+* FP32 debug path vs the FP64 oracle: forward and every gradient <= 1e-6; the
+  loss (an FP32 atomic sum of 3000 terms) <= 1e-5;
+* BF16 / MFMA path vs the oracle's BF16 rounding model (inputs, activations
+  and deltas rounded where the kernels round them, FP64 accumulation):
+  forward <= 1e-3, every layer's and the grid's gradient <= 5e-3;
+* BF16 path vs the exact FP64 oracle: forward and loss <= 2e-2, MLP layers
+  <= 3e-2; the grid gradient is printed with its conditioning (sum of |terms|
+  over |sum|): its entries cancel heavily, so rounding to 8 bits shows there
+  as a large relative error of a small sum;
 * training on a smooth synthetic radiance field reduces the validation loss 10x;
-* the cached renderer produces finite images and trains on its own samples.
+* the cached renderer produces finite images and trains on its own samples;
+* at equal spp the trained cache beats the plain tracer and the untrained cache.
 """
 import numpy as np
 import pytest
@@ -92,7 +95,7 @@ def test_forward_and_gradients_match_fp64_oracle(gpu_ctx, precision, fwd_tol, gr
     print(f"[cache {name}] forward {ef:.2e}, loss {el:.2e}, layers {[f'{e:.2e}' for e in layer_err]}, grid {eg:.2e}",
           flush=True)
     assert ef <= fwd_tol
-    assert el <= fwd_tol
+    assert el <= max(fwd_tol, 1e-5)
     if grad_tol is not None:
         assert max(layer_err) <= grad_tol, layer_err
         assert eg <= grad_tol
@@ -181,8 +184,75 @@ def test_cached_render_trains_on_its_own_paths(gpu_ctx):
     assert np.all(im[..., 3] == frames)
     assert im[..., :3].mean() > 0
     for s in stats:
-        assert 0 < s["train_samples"] <= 2048 * 4
+        assert 0 < s["train_samples"] <= 2 * 2048 * 4
         assert 0 < s["queries"] <= w * h * spp
         assert np.isfinite(s["loss"])
     assert c.info()["steps"] == 2 * frames
     c.close()
+
+
+def test_trained_cache_gains_over_the_plain_tracer(gpu_ctx):
+    """Terrain with 200 mixed-material spheres, a 2048x1024 sky with a sun
+    lobe of peak 1e4, 240x136, paths of depth 12 with RR from 4.  Reference:
+    the plain tracer at 4096 spp.  At 256 spp each:
+    * ending paths at vertex 2 removes deep-path variance whether or not the
+      cache is trained: both cached images beat the plain tracer's relMSE;
+    * what the trained network adds is the light those paths would have
+      gathered: the untrained cache leaves the image > 8% too dark and its
+      low-frequency error (relMSE of 8x8 box-filtered images) far above the
+      trained cache's, which is below the plain tracer's."""
+    import torch
+    from voidray_amd import workloads as W
+    from voidray_amd.device import make_render_desc, scene_bounds
+    from voidray_amd.scene import RenderSettings
+    announce("cache gain: terrain 240x136 at 256 spp, plain vs trained vs untrained cache (vertex 2)")
+    scene = W.terrain_spheres(grid=256, n_spheres=200, sphere_res=(32, 16), tex_size=512, env_size=(2048, 1024))
+    accel = scene.build_acceleration(gpu_ctx)
+    w, h, spp, spf, warm = 240, 136, 256, 4, 16
+    st = RenderSettings(total_samples=1, max_bounces=12, firefly_clamp=float("inf"))
+    flags = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+    img = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def plain(n, seed):
+        img.zero_()
+        torch.cuda.synchronize()
+        for k in range(0, n, 64):
+            accel.render_device(img.data_ptr(), make_render_desc(w, h, min(64, n - k), n, st, sample_offset=k,
+                                                                 seed=seed, flags=flags, rr_depth=4))
+        torch.cuda.synchronize()
+        return img.cpu().numpy()[..., :3].astype(np.float64)
+
+    def relmse(x, r):
+        return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+    def box8(x):
+        return x.reshape(h // 8, 8, w // 8, 8, 3).mean((1, 3))
+
+    ref = plain(4096, seed=99)
+    res = {"plain": plain(spp, seed=7)}
+    lo, hi = scene_bounds(scene)
+    scratch = torch.zeros_like(img)
+    for trained in (True, False):
+        c = RadianceCache(gpu_ctx, lo, hi, log2_table=16, base_resolution=8, max_resolution=512, seed=1)
+        img.zero_()
+        torch.cuda.synchronize()
+        for f in range(warm + spp // spf):
+            # warm-up frames train the cache; their image is discarded
+            target, off = (scratch, 0) if f < warm else (img, (f - warm) * spf)
+            c.render_frame(accel, target.data_ptr(), make_render_desc(w, h, spf, spp, st, sample_offset=off, seed=7,
+                                                                      flags=flags, rr_depth=4),
+                           train_paths=1 << 13, train_vertices=4, cache_depth=2, train_steps=8 if trained else 0,
+                           frame=f)
+        c.stats()
+        torch.cuda.synchronize()
+        res["trained" if trained else "untrained"] = img.cpu().numpy()[..., :3].astype(np.float64)
+        assert c.info()["steps"] == (8 * (warm + spp // spf) if trained else 0)
+        c.close()
+    m = {k: (relmse(v, ref), float(v.mean() / ref.mean()), relmse(box8(v), box8(ref))) for k, v in res.items()}
+    for k, (e, mr, eb) in m.items():
+        print(f"[cache gain] {k:9s}: relMSE {e:.4f}, mean / reference {mr:.4f}, 8x8 relMSE {eb:.5f}", flush=True)
+    assert m["trained"][0] < 0.75 * m["plain"][0] and m["untrained"][0] < 0.75 * m["plain"][0]
+    assert abs(m["trained"][1] - 1.0) < 0.05 and m["untrained"][1] < 0.92
+    assert m["trained"][2] < 0.35 * m["untrained"][2]
+    assert m["trained"][2] < 1.2 * m["plain"][2]

@@ -69,7 +69,7 @@ struct vr_cache {
     // pinned read-back: [0] train count, [1] query count, [2] mean loss (f32 bits)
     uint32_t* host_rb = nullptr;
     hipEvent_t done = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // frame: start, training done, render done
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // frame: start, paths done, training done
     bool done_recorded = false, frame_timed = false;
 };
 
@@ -78,9 +78,11 @@ int nrc_infer(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* pos, co
               const float* rough, float* out, hipStream_t st);
 int nrc_frame_readback(vr_cache* c, hipStream_t st);
 // Loss gradients into the cache's gradient buffers (zero on entry, zero again
-// after an update); update = Adam step.
+// after an update); update = Adam step.  Mini-batch `part` of `parts` equal
+// slices of the samples.
 int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* pos, const float* dir, const float* nrm,
-                   const float* rough, const float* target, bool update, hipStream_t st);
+                   const float* rough, const float* target, bool update, hipStream_t st, uint32_t part = 0,
+                   uint32_t parts = 1);
 int nrc_ensure_frame_buffers(vr_cache* c, uint32_t train_cap, uint32_t query_cap);
 
 }  // namespace vr

@@ -207,7 +207,16 @@ struct MlpArgs {
     __bf16 *xT, *hT, *dT; // training: transposed [rows][cap]
     uint32_t cap;
     float* grid_grad;
+    uint32_t part, parts;  // training mini-batch: slice part of parts of the samples
 };
+
+// Mini-batch `part` of `parts`: samples [begin, begin + count) of total.
+__device__ __forceinline__ void slice_of(uint32_t total, uint32_t part, uint32_t parts, uint32_t& begin,
+                                         uint32_t& count) {
+    const uint64_t b = (uint64_t)total * part / parts, e = (uint64_t)total * (part + 1) / parts;
+    begin = (uint32_t)b;
+    count = (uint32_t)(e - b);
+}
 
 // Copies the weights into LDS with padded rows.
 __device__ __forceinline__ void load_weights(const __bf16* src, __bf16* W) {
@@ -230,7 +239,14 @@ template <bool TRAIN>
 __device__ __forceinline__ void nrc_mlp_body(const MlpArgs& a, __bf16* W) {
     load_weights(a.v.w_bf, W);
     __syncthreads();
-    const uint32_t n = a.d_n ? min(*a.d_n, a.n) : a.n;  // a device count never exceeds the capacity a.n
+    const uint32_t total = a.d_n ? min(*a.d_n, a.n) : a.n;  // a device count never exceeds the capacity a.n
+    uint32_t begin = 0, n = total;
+    if (TRAIN) slice_of(total, a.part, a.parts, begin, n);
+    const float* pos = a.pos + 3 * (size_t)begin;
+    const float* dir = a.dir + 3 * (size_t)begin;
+    const float* nrm = a.nrm + 3 * (size_t)begin;
+    const float* rough = a.rough + begin;
+    const float* target = TRAIN ? a.target + 3 * (size_t)begin : nullptr;
     const int lane = threadIdx.x & 63, col = lane & 15, q = lane >> 4, wave = threadIdx.x >> 6;
     const uint32_t ntiles = TRAIN ? ((n + 31) / 32) * 2 : (n + 15) / 16;
     const float scale = n ? 1.f / (3.f * (float)n) : 0.f;
@@ -239,7 +255,7 @@ __device__ __forceinline__ void nrc_mlp_body(const MlpArgs& a, __bf16* W) {
         const uint32_t s = tile * 16 + col;
         const bool valid = s < n;
         EncIn e{};
-        if (valid) e = enc_prepare(a.v, a.pos, a.dir, a.nrm, a.rough, s);
+        if (valid) e = enc_prepare(a.v, pos, dir, nrm, rough, s);
         bf16x8 xb[4];
         for (int li = 0; li < 4; ++li) {
             float f0 = 0.f, f1 = 0.f;
@@ -279,7 +295,7 @@ __device__ __forceinline__ void nrc_mlp_body(const MlpArgs& a, __bf16* W) {
         float lsum = 0.f;
         if (q == 0 && valid) {
             for (int i = 0; i < 3; ++i) {
-                float t = a.target[3 * (size_t)s + i];
+                float t = target[3 * (size_t)s + i];
                 float den = y[i] * y[i] + 0.01f;
                 float diff = y[i] - t;
                 lsum += diff * diff / den;
@@ -355,9 +371,11 @@ __global__ __launch_bounds__(kFwdBlock) void k_nrc_train(MlpArgs a) {
 // dW_L = sum over samples of delta_L (x) input_L, split over blocks along the
 // samples; blockIdx.y = layer.  Partials are reduced by k_nrc_reduce.
 __global__ __launch_bounds__(256) void k_nrc_wgrad(const __bf16* xT, const __bf16* hT, const __bf16* dT, uint32_t cap,
-                                                   const uint32_t* d_n, uint32_t n_host, float* part) {
+                                                   const uint32_t* d_n, uint32_t n_host, uint32_t mb, uint32_t mbs,
+                                                   float* part) {
     const int layer = blockIdx.y;
-    const uint32_t n = d_n ? min(*d_n, n_host) : n_host;
+    uint32_t begin = 0, n = 0;
+    slice_of(d_n ? min(*d_n, n_host) : n_host, mb, mbs, begin, n);
     const uint32_t npad = (n + 31) & ~31u;
     const int M = layer == 4 ? 16 : 64, K = layer == 0 ? 128 : 64;
     const int offs[5] = {kNrcOff0, kNrcOff1, kNrcOff2, kNrcOff3, kNrcOff4};
@@ -404,10 +422,17 @@ __global__ void k_nrc_reduce(const float* part, uint32_t blocks, float* grad) {
 // All-FP32 debug path: one thread per sample, weights and grid from the FP32
 // masters, gradients by atomics.
 __global__ void k_nrc_f32(MlpArgs a, int train, float* w_grad) {
-    const uint32_t n = a.d_n ? min(*a.d_n, a.n) : a.n;  // a device count never exceeds the capacity a.n
+    const uint32_t total = a.d_n ? min(*a.d_n, a.n) : a.n;  // a device count never exceeds the capacity a.n
+    uint32_t begin = 0, n = total;
+    if (train) slice_of(total, a.part, a.parts, begin, n);
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const NrcView& v = a.v;
+    a.pos += 3 * (size_t)begin;
+    a.dir += 3 * (size_t)begin;
+    a.nrm += 3 * (size_t)begin;
+    a.rough += begin;
+    if (train) a.target += 3 * (size_t)begin;
     const float* Wf = v.w_f;
     EncIn e = enc_prepare(v, a.pos, a.dir, a.nrm, a.rough, s);
     float x[kNrcIn];
@@ -485,10 +510,12 @@ __global__ void k_nrc_f32(MlpArgs a, int train, float* w_grad) {
 }
 
 // Adam (Kingma & Ba) with the bias corrections folded into lr_t; clears the gradient.
+// lazy: entries with a zero gradient are left untouched (their moments are not decayed).
 __global__ void k_adam(float* p, float* g, float* m, float* v, uint64_t n, float lr_t, float b1, float b2, float eps,
-                       __bf16* out_bf, _Float16* out_h) {
+                       __bf16* out_bf, _Float16* out_h, int lazy) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float gi = g[i];
+        if (lazy && gi == 0.f) continue;
         float mi = b1 * m[i] + (1.f - b1) * gi;
         float vi = b2 * v[i] + (1.f - b2) * (gi * gi);
         float pi = p[i] - lr_t * mi / (sqrtf(vi) + eps);
@@ -514,9 +541,10 @@ __global__ void k_fill_f32(float* p, uint64_t n, float value) {
 }
 
 // loss[1] = loss[0] / (3 n)
-__global__ void k_loss_finish(float* loss, const uint32_t* d_n, uint32_t n_host) {
+__global__ void k_loss_finish(float* loss, const uint32_t* d_n, uint32_t n_host, uint32_t mb, uint32_t mbs) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        uint32_t n = d_n ? min(*d_n, n_host) : n_host;
+        uint32_t begin = 0, n = 0;
+        slice_of(d_n ? min(*d_n, n_host) : n_host, mb, mbs, begin, n);
         loss[1] = n ? loss[0] / (3.f * (float)n) : 0.f;
     }
 }
@@ -591,7 +619,9 @@ int nrc_infer(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* pos, co
 }
 
 int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* pos, const float* dir, const float* nrm,
-                   const float* rough, const float* target, bool update, hipStream_t st) {
+                   const float* rough, const float* target, bool update, hipStream_t st, uint32_t part,
+                   uint32_t parts) {
+    if (parts == 0 || part >= parts) return fail(VR_ERR_INVALID, "bad mini-batch index");
     int rc = ensure_train_scratch(c, std::max(n, 1u));
     if (rc) return rc;
     rc = fill(c->loss, 2, 0.f, st);
@@ -600,6 +630,8 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
     a.target = target;
     a.loss = c->loss;
     a.grid_grad = c->g_grad;
+    a.part = part;
+    a.parts = parts;
     if (n) {
         if (c->view.fp32) {
             hipLaunchKernelGGL(k_nrc_f32, dim3((n + 127) / 128), dim3(128), 0, st, a, 1, c->w_grad);
@@ -610,12 +642,13 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
             a.cap = c->train_cap;
             hipLaunchKernelGGL(k_nrc_train, dim3(mlp_blocks(n)), dim3(kFwdBlock), 0, st, a);
             hipLaunchKernelGGL(k_nrc_wgrad, dim3(kWgradBlocks, 5), dim3(256), 0, st, (const __bf16*)c->xT,
-                               (const __bf16*)c->hT, (const __bf16*)c->dT, c->train_cap, d_n, n, c->part);
+                               (const __bf16*)c->hT, (const __bf16*)c->dT, c->train_cap, d_n, n, part, parts,
+                               c->part);
             hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 255) / 256), dim3(256), 0, st, (const float*)c->part,
                                (uint32_t)kWgradBlocks, c->w_grad);
         }
     }
-    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, c->loss, d_n, n);
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, c->loss, d_n, n, part, parts);
     VR_HIP_CHECK(hipGetLastError());
     if (!update) return 0;
     c->info.steps += 1;
@@ -625,10 +658,10 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
     const float lrw = (float)(c->desc.lr_mlp * corr), lrg = (float)(c->desc.lr_grid * corr);
     hipLaunchKernelGGL(k_adam, dim3(grid_launch(kNrcParams)), dim3(256), 0, st, c->w_master, c->w_grad, c->w_m,
                        c->w_v, (uint64_t)kNrcParams, lrw, (float)b1, (float)b2, c->desc.eps, c->w_bf,
-                       (_Float16*)nullptr);
+                       (_Float16*)nullptr, 0);
     hipLaunchKernelGGL(k_adam, dim3(grid_launch(c->info.grid_params)), dim3(256), 0, st, c->g_master, c->g_grad,
                        c->g_m, c->g_v, c->info.grid_params, lrg, (float)b1, (float)b2, c->desc.eps, (__bf16*)nullptr,
-                       c->g_h);
+                       c->g_h, 1);
     VR_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -880,7 +913,7 @@ int vr_cache_gradients_async(vr_cache* c, uint32_t n, const float* d_pos, const 
         if (!c || (n && (!d_pos || !d_dir || !d_nrm || !d_rough || !d_target)))
             return fail(VR_ERR_INVALID, "null argument");
         hipStream_t st = pick(c, stream);
-        int rc = nrc_train_step(c, n, nullptr, d_pos, d_dir, d_nrm, d_rough, d_target, false, st);
+        int rc = nrc_train_step(c, n, nullptr, d_pos, d_dir, d_nrm, d_rough, d_target, false, st, 0, 1);
         if (rc) return rc;
         const uint64_t G = c->info.grid_params;
         if (d_mlp_grad) VR_HIP_CHECK(hipMemcpyAsync(d_mlp_grad, c->w_grad, 4 * kNrcParams, hipMemcpyDeviceToDevice, st));
@@ -898,7 +931,7 @@ int vr_cache_train_async(vr_cache* c, uint32_t n, const float* d_pos, const floa
         if (!c || (n && (!d_pos || !d_dir || !d_nrm || !d_rough || !d_target)))
             return fail(VR_ERR_INVALID, "null argument");
         hipStream_t st = pick(c, stream);
-        int rc = nrc_train_step(c, n, nullptr, d_pos, d_dir, d_nrm, d_rough, d_target, true, st);
+        int rc = nrc_train_step(c, n, nullptr, d_pos, d_dir, d_nrm, d_rough, d_target, true, st, 0, 1);
         if (rc) return rc;
         VR_HIP_CHECK(hipMemcpyAsync(c->host_rb + 2, c->loss + 1, 4, hipMemcpyDeviceToHost, st));
         VR_HIP_CHECK(hipEventRecord(c->done, st));
@@ -921,8 +954,8 @@ int vr_cache_last_stats(vr_cache* c, vr_cache_stats* stats) {
         stats->queries = c->host_rb[1];
         if (c->frame_timed) {
             float t = 0.f, r = 0.f;
-            VR_HIP_CHECK(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
-            VR_HIP_CHECK(hipEventElapsedTime(&r, c->ev[1], c->ev[2]));
+            VR_HIP_CHECK(hipEventElapsedTime(&r, c->ev[0], c->ev[1]));  // the frame's paths
+            VR_HIP_CHECK(hipEventElapsedTime(&t, c->ev[1], c->ev[2]));  // Adam steps on its samples
             stats->train_ms = t;
             stats->render_ms = r;
         }

@@ -6,11 +6,11 @@ namespace vr {
 int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, hipStream_t st);
 // Radiance-cache role of one request of a batch.
 struct CacheReq {
-    uint32_t cmode;           // 0 none, 1 training paths, 2 cache queries
-    uint32_t cache_depth;     // query vertex
-    uint32_t train_vertices;  // recorded vertices per training path
-    uint32_t train_paths;     // training paths (replace the image's pixels)
-    uint32_t train_stride, train_offset;
+    uint32_t cmode;            // 0 none, 1 cached
+    uint32_t cache_depth;      // query vertex
+    uint32_t train_vertices;   // recorded vertices per training path
+    uint32_t train_threshold;  // fraction of training paths * 2^32
+    uint32_t frame;
 };
 // Plans, uploads and enqueues a batch (optionally captured into a graph and launched).
 int render_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning* tuning, uint32_t flags,

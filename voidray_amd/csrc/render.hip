@@ -40,7 +40,12 @@ namespace vr {
 constexpr uint32_t kDefaultChunkPaths = 1u << 22;
 constexpr uint32_t kMaxChunkPaths = 1u << 26;
 constexpr uint32_t kMaxBouncesLimit = 64;
-constexpr uint32_t kShadeGrid = 2048;
+constexpr uint32_t kShadeGrid = 2048;  // most blocks a grid-stride kernel over a chunk launches
+
+// Blocks of a grid-stride kernel over at most n items.
+static inline uint32_t grid_for(uint32_t n, uint32_t block) {
+    return std::max(1u, std::min(kShadeGrid, (n + block - 1) / block));
+}
 
 // render counters (u32 words per chunk)
 enum : int {
@@ -51,8 +56,10 @@ enum : int {
     kCntWords
 };
 
-// radiance-cache role of a request
-enum : uint32_t { kCacheNone = 0, kCacheTrain = 1, kCacheQuery = 2 };
+// radiance-cache mode of a request: paths stop at the query vertex and take
+// the cache's answer, except a random subset of training paths that run to the
+// end and record their first vertices as training samples
+enum : uint32_t { kCacheNone = 0, kCacheOn = 1 };
 
 struct ReqDev {
     float* rgba;
@@ -61,11 +68,12 @@ struct ReqDev {
     int32_t mode;
     float clamp;
     uint32_t ext;
-    uint32_t npix;            // paths / spp: W*H, or the number of training paths
+    uint32_t npix;            // W*H
     uint32_t cmode;           // kCache*
     uint32_t cache_depth;     // query vertex
     uint32_t train_vertices;  // recorded vertices per training path
-    uint32_t train_stride, train_offset;  // training path k traces pixel (k*stride + offset) mod W*H
+    uint32_t train_threshold; // a path trains iff train_hash < threshold (threshold / 2^32 = fraction)
+    uint32_t frame;
 };
 
 struct SegDev {
@@ -142,6 +150,18 @@ __device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
     return want ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
 }
 
+// Training-path selection: a 32-bit mix of (pixel, sample, request, frame)
+// (the murmur3 finaliser), compared with the request's threshold.
+__device__ __forceinline__ uint32_t train_hash(uint32_t pixel, uint32_t sample, uint32_t request, uint32_t frame) {
+    uint32_t h = pixel * 0x9E3779B1u ^ (sample + 0x7F4A7C15u) * 0x85EBCA77u ^ request * 0xC2B2AE3Du ^ frame * 0x27D4EB2Fu;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
 // Roughness input of the radiance cache: 1 for diffuse and emissive
 // surfaces, the fuzz of Metal, 0 for Dielectric.
 __device__ __forceinline__ float cache_roughness(const GpuMaterial& m) {
@@ -165,8 +185,6 @@ __global__ void k_raygen(RenderArgs a) {
     uint32_t local = p - sg.path_base;
     uint32_t i = sg.pix0 + local / r.spp;
     uint32_t s = r.sample_offset + local % r.spp;
-    if (r.cmode == kCacheTrain)  // training paths spread over the image by a stride coprime to W*H
-        i = (uint32_t)(((uint64_t)i * r.train_stride + r.train_offset) % ((uint64_t)r.W * r.H));
     a.path_req[p] = sg.req;
     a.path_pix[p] = i;
     a.path_smp[p] = s;
@@ -284,8 +302,11 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                     }
                 } else {
                     vf3 A = ld3(a.A, p), L = ld3(a.L, p);
-                    const bool query = r.cmode == kCacheQuery && depth == r.cache_depth;
-                    if (query || (r.cmode == kCacheTrain && depth < r.train_vertices)) {
+                    const bool train_path =
+                        r.cmode == kCacheOn &&
+                        train_hash(a.path_pix[p], a.path_smp[p], r.request, r.frame) < r.train_threshold;
+                    const bool query = r.cmode == kCacheOn && !train_path && depth == r.cache_depth;
+                    if (query || (train_path && depth < r.train_vertices)) {
                         // the vertex, its outgoing direction and the path state before its contributions
                         vf3 ns = m.normal_tex >= 0 ? tex_sample(sv, m.normal_tex, hit.uv.x, hit.uv.y) : hit.normal;
                         rec = query ? 2u : 1u;
@@ -392,9 +413,11 @@ __global__ void k_cache_apply(RenderArgs a) {
 
 // Training targets: the radiance a path gathered from vertex k on, divided by
 // the throughput that arrived at vertex k, (L_final - L_k) / A_k per channel.
+// Samples of this chunk: [count[1], count[0]) (count[1] marks the chunk start).
 __global__ void k_cache_targets(RenderArgs a) {
-    const uint32_t n = min(*a.tr.count, a.tr.cap);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t n = min(a.tr.count[0], a.tr.cap);
+    const uint32_t k0 = min(a.tr.count[1], n);
+    for (uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t p = a.tr.path[k];
         vf3 lf = ld3(a.L, p), lk = ld3(a.tr.L, k), ak = ld3(a.tr.A, k);
         float t[3];
@@ -406,6 +429,10 @@ __global__ void k_cache_targets(RenderArgs a) {
     }
 }
 
+__global__ void k_cache_mark(uint32_t* count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) count[1] = count[0];
+}
+
 // Per pixel: sum of the samples in index order, scaled by 1/total_samples
 // and added into the request's target (iterative.rs:45-51).
 __global__ void k_accumulate(RenderArgs a) {
@@ -414,7 +441,6 @@ __global__ void k_accumulate(RenderArgs a) {
     uint32_t k = seg_find(a.segs, a.nseg, q, [](const SegDev& s) { return s.pix_base; });
     const SegDev sg = a.segs[k];
     const ReqDev& r = a.reqs[sg.req];
-    if (r.cmode == kCacheTrain) return;  // training paths feed the cache, not the image
     uint32_t local = q - sg.pix_base;
     uint32_t base = sg.path_base + local * r.spp;
     vf3 color = mk3(0.f, 0.f, 0.f);
@@ -523,7 +549,7 @@ struct Group {
     vr_scene* scene = nullptr;
     uint32_t engine = 0;
     bool any_ext = false, any_env_is = false;
-    bool any_train = false, any_query = false;  // radiance-cache roles
+    bool any_cached = false;  // radiance-cache requests (queries and training samples)
     uint32_t att_levels = 0;  // max bounces among reference-estimator requests
     uint32_t max_depth = 0;
     uint64_t paths = 0;
@@ -633,9 +659,8 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
             r.cmode = creq[i].cmode;
             r.cache_depth = creq[i].cache_depth;
             r.train_vertices = creq[i].train_vertices;
-            r.train_stride = creq[i].train_stride;
-            r.train_offset = creq[i].train_offset;
-            if (r.cmode == kCacheTrain) r.npix = d.width * d.height ? creq[i].train_paths : 0;
+            r.train_threshold = creq[i].train_threshold;
+            r.frame = creq[i].frame;
         }
         R.reqs.push_back(r);
         uint64_t rp = (uint64_t)r.npix * d.samples;
@@ -656,12 +681,9 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         g.any_ext |= r.ext != 0;
         g.any_env_is |= r.ext && (d.flags & VR_RENDER_ENV_IS) && it.scene->view.env_kind == VR_ENV_HDRI;
         if (!r.ext) g.att_levels = std::max(g.att_levels, d.max_bounces);
-        // query paths end at the query vertex
-        uint32_t depth = r.cmode == kCacheQuery ? std::min(d.max_bounces, r.cache_depth + 1) : d.max_bounces;
-        g.max_depth = std::max(g.max_depth, depth);
+        g.max_depth = std::max(g.max_depth, d.max_bounces);
         g.paths += rp;
-        g.any_train |= r.cmode == kCacheTrain;
-        g.any_query |= r.cmode == kCacheQuery;
+        g.any_cached |= r.cmode == kCacheOn;
     }
     if (flags & VR_BATCH_SHORTEST_FIRST) {
         std::stable_sort(R.groups.begin(), R.groups.end(),
@@ -822,10 +844,12 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
         a.queue[0] = carve<uint32_t>(cur, cap);
         a.queue[1] = carve<uint32_t>(cur, cap);
         a.cnt = carve<uint32_t>(cur, kCntWords);
-        if (g.any_train || g.any_query) {
+        if (g.any_cached) {
             if (!R.cache) return fail(VR_ERR_INVALID, "radiance-cache request without a cache");
-            if (g.any_train) a.tr = R.cache->tr;
-            if (g.any_query) a.qr = R.cache->qr;
+            a.tr = R.cache->tr;
+            a.qr = R.cache->qr;
+            // training samples accumulate over the group's chunks
+            hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.tr.count, 2u);
         }
         for (const Chunk& c : g.chunks) {
             if (c.npaths) {
@@ -835,8 +859,10 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                 a.npaths = c.npaths;
                 hipLaunchKernelGGL(k_render_zero, dim3((kCntWords + 255) / 256), dim3(256), 0, st, a.cnt,
                                    (uint32_t)kCntWords);
-                if (g.any_train) hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.tr.count, 1u);
-                if (g.any_query) hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.qr.count, 1u);
+                if (g.any_cached) {
+                    hipLaunchKernelGGL(k_cache_mark, dim3(1), dim3(64), 0, st, a.tr.count);
+                    hipLaunchKernelGGL(k_render_zero, dim3(1), dim3(64), 0, st, a.qr.count, 1u);
+                }
                 hipLaunchKernelGGL(k_raygen, dim3((c.npaths + sb - 1) / sb), dim3(sb), 0, st, a);
                 for (uint32_t depth = 0; depth < g.max_depth; ++depth) {
                     TraceJob j;
@@ -855,7 +881,7 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                     j.fallback_accum = R.stats_dev + 3;
                     int rc = launch_trace(s, j, st);
                     if (rc) return rc;
-                    hipLaunchKernelGGL(k_shade, dim3(kShadeGrid), dim3(sb), 0, st, a, depth);
+                    hipLaunchKernelGGL(k_shade, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a, depth);
                     if (g.any_env_is) {
                         TraceJob sj;
                         sj.o = a.sh_o;
@@ -871,16 +897,17 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                         sj.fallback_accum = R.stats_dev + 3;
                         rc = launch_trace(s, sj, st);
                         if (rc) return rc;
-                        hipLaunchKernelGGL(k_shadow_resolve, dim3(kShadeGrid), dim3(sb), 0, st, a, depth);
+                        hipLaunchKernelGGL(k_shadow_resolve, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a,
+                                           depth);
                     }
                 }
-                if (g.any_query) {
+                if (g.any_cached) {
                     int rc = nrc_infer(R.cache, a.qr.cap, a.qr.count, a.qr.pos, a.qr.dir, a.qr.nrm, a.qr.rough,
                                        a.qr.target, st);
                     if (rc) return rc;
-                    hipLaunchKernelGGL(k_cache_apply, dim3(kShadeGrid), dim3(sb), 0, st, a);
+                    hipLaunchKernelGGL(k_cache_apply, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a);
+                    hipLaunchKernelGGL(k_cache_targets, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a);
                 }
-                if (g.any_train) hipLaunchKernelGGL(k_cache_targets, dim3(kShadeGrid), dim3(sb), 0, st, a);
                 hipLaunchKernelGGL(k_accumulate, dim3((c.npix + sb - 1) / sb), dim3(sb), 0, st, a);
                 hipLaunchKernelGGL(k_stats_chunk, dim3(1), dim3(64), 0, st, (const uint32_t*)a.cnt, R.stats_dev,
                                    c.npaths, g.max_depth);
@@ -952,17 +979,10 @@ int render_device(vr_scene* s, const vr_render_desc* desc, float* d_rgba, hipStr
     return render_batch(s->ctx, &r, 1, nullptr, 0, st);
 }
 
-static uint32_t gcd32(uint32_t a, uint32_t b) {
-    while (b) {
-        uint32_t t = a % b;
-        a = b;
-        b = t;
-    }
-    return a;
-}
-
-// One frame of the cached renderer: training paths -> samples -> Adam steps,
-// then render paths that stop at the query vertex and take the cache's answer.
+// One frame of the cached renderer: the frame's paths stop at the query
+// vertex and take the cache's answer, except ~train_paths randomly chosen
+// training paths that run to the end (and count in the image like any path)
+// and record their first vertices as samples; then Adam steps on the samples.
 int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr_cache_frame* f, float* d_rgba,
                   hipStream_t st) {
     vr_ctx* ctx = s->ctx;
@@ -971,47 +991,34 @@ int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr
         return fail(VR_ERR_INVALID, "the cached renderer runs the extended estimator (flags ENV_IS and/or RUSSIAN_ROULETTE)");
     if (f->train_vertices == 0 || f->train_vertices > kNrcMaxTrainVertices)
         return fail(VR_ERR_INVALID, "train_vertices must be in [1, 8]");
-    if (f->train_paths > kMaxChunkPaths) return fail(VR_ERR_INVALID, "train_paths above 2^26");
     if (f->cache_depth >= desc->max_bounces) return fail(VR_ERR_INVALID, "cache_depth must be below max_bounces");
     const uint64_t npx = (uint64_t)desc->width * desc->height;
-    if (npx == 0) return 0;
+    if (npx == 0 || desc->samples == 0) return 0;
     if (npx >= (1ull << 32)) return fail(VR_ERR_INVALID, "image too large");
-    const uint64_t train_cap = (uint64_t)f->train_paths * f->train_vertices;
+    const uint64_t paths = npx * desc->samples;
+    const uint64_t train_paths = std::min<uint64_t>(f->train_paths, paths);
+    // room for twice the expected number of samples; a frame's excess is dropped
+    const uint64_t train_cap = std::max<uint64_t>(64, 2 * train_paths * f->train_vertices);
     if (train_cap > (1ull << 28)) return fail(VR_ERR_INVALID, "too many training samples per frame");
-    const uint32_t qcap = (uint32_t)std::min<uint64_t>(npx * std::max(desc->samples, 1u), kDefaultChunkPaths);
+    const uint32_t qcap = (uint32_t)std::min<uint64_t>(paths, kDefaultChunkPaths);
     int rc = ensure_state(ctx);
     if (!rc) rc = nrc_ensure_frame_buffers(c, (uint32_t)train_cap, qcap);
     if (rc) return rc;
-    VR_HIP_CHECK(hipEventRecord(c->ev[0], st));
-    if (f->train_paths && f->train_steps) {
-        vr_request tr{s, *desc, d_rgba};
-        tr.desc.samples = 1;
-        tr.desc.total_samples = 1;
-        tr.desc.sample_offset = f->frame;
-        tr.desc.request = desc->request ^ 0x80000000u;  // streams apart from the image's paths
-        CacheReq cq{};
-        cq.cmode = kCacheTrain;
-        cq.train_vertices = f->train_vertices;
-        cq.train_paths = f->train_paths;
-        uint32_t stride = (uint32_t)(2654435761ull % npx);
-        if (stride == 0) stride = 1;
-        while (gcd32(stride, (uint32_t)npx) != 1) ++stride;
-        cq.train_stride = stride;
-        cq.train_offset = (uint32_t)(((uint64_t)f->frame * 2246822519ull + 12345ull) % npx);
-        vr_tuning t{std::max<uint32_t>(kDefaultChunkPaths, (f->train_paths + 63) & ~63u), 0};
-        rc = render_batch(ctx, &tr, 1, &t, 0, st, &cq, c);
-        if (rc) return rc;
-        for (uint32_t k = 0; k < f->train_steps && !rc; ++k)
-            rc = nrc_train_step(c, c->tr.cap, c->tr.count, c->tr.pos, c->tr.dir, c->tr.nrm, c->tr.rough, c->tr.target,
-                                true, st);
-        if (rc) return rc;
-    }
-    VR_HIP_CHECK(hipEventRecord(c->ev[1], st));
     CacheReq cr{};
-    cr.cmode = kCacheQuery;
+    cr.cmode = kCacheOn;
     cr.cache_depth = f->cache_depth;
+    cr.train_vertices = f->train_vertices;
+    cr.train_threshold = (uint32_t)std::min<double>(4294967295.0, (double)train_paths / (double)paths * 4294967296.0);
+    cr.frame = f->frame;
+    VR_HIP_CHECK(hipEventRecord(c->ev[0], st));
     vr_request rq{s, *desc, d_rgba};
     rc = render_batch(ctx, &rq, 1, nullptr, 0, st, &cr, c);
+    if (rc) return rc;
+    VR_HIP_CHECK(hipEventRecord(c->ev[1], st));
+    // one Adam step per mini-batch: the frame's samples split into train_steps slices
+    for (uint32_t k = 0; k < f->train_steps && train_paths && !rc; ++k)
+        rc = nrc_train_step(c, c->tr.cap, c->tr.count, c->tr.pos, c->tr.dir, c->tr.nrm, c->tr.rough, c->tr.target, true,
+                            st, k, f->train_steps);
     if (rc) return rc;
     VR_HIP_CHECK(hipEventRecord(c->ev[2], st));
     c->frame_timed = true;
