@@ -8,7 +8,9 @@ namespace vr {
 constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
-constexpr int kRefill = 16;      // idle lanes that trigger a wave-wide refill
+constexpr int kRefill = 16;      // idle lanes that trigger a wave-wide refill (closest hit)
+constexpr int kRefillAny = 32;   // the same for any-hit queries (shorter rays: refill less often)
+constexpr int kNodeSteps = 3;    // most node-phase iterations between two leaf phases
 
 struct TraceArgs {
     SceneView sv;
@@ -40,10 +42,11 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
     if (a.hit) a.hit[i] = h.prim != 0xFFFFFFFFu ? 1 : 0;
 }
 
-// Persistent wide-engine kernel: every lane runs its own ray as a state
-// machine, one node or leaf per iteration; whenever kRefill lanes of the wave
-// are idle the wave fetches that many new rays with one atomic (dynamic ray
-// fetch), so long incoherent rays do not leave the rest of the wave idle.
+// Persistent wide-engine kernel: every lane runs its own ray as a resumable
+// state machine; each iteration is a node phase (up to kNodeSteps inner nodes)
+// followed by a leaf phase (one leaf's triangles); whenever kRefill lanes of
+// the wave are idle the wave fetches that many new rays with one atomic
+// (dynamic ray fetch), so long incoherent rays do not leave the wave idle.
 template <bool ANY>
 __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     __shared__ u32x2 lds[kLdsStack * kWave];
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
         bool idle = ray == kNone;
         uint64_t idle_mask = __ballot(idle);
         int nidle = __popcll(idle_mask);
-        if (!exhausted && nidle >= kRefill) {
+        if (!exhausted && nidle >= (ANY ? kRefillAny : kRefill)) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(a.next, (uint32_t)nidle);
             base = __shfl(base, 0);
@@ -76,19 +79,48 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
             }
         }
         if (exhausted && __ballot(ray != kNone) == 0ull) break;
-        if (ray != kNone) {
-            int rc = wide_step<kLdsStack, ANY>(a.sv, st, ws, a.tmin, a.tmax);
-            if (rc != kStepContinue) {
-                HitOut h;
-                if (rc == kStepDone) rc = wide_finish<ANY>(a.sv, st, a.tmin, h);
-                if (rc == kStepFallback) {
-                    uint32_t slot = atomicAdd(a.fb_count, 1u);
-                    a.fb_list[slot] = ray;
-                } else {
-                    write_out(a, ray, h);
+        // Node phase (while-while traversal with leaf postponement, Aila &
+        // Laine 2009): lanes walk inner nodes together; a lane that reaches a
+        // leaf parks it and keeps walking until it meets a second leaf; the
+        // phase ends once no lane is still looking for its first leaf.  Node
+        // and triangle work then run in separate passes instead of splitting
+        // every iteration between the two.
+        const bool live = ray != kNone;
+        uint32_t leaf = kNone;
+        bool done = false, fallback = false;
+        for (int it = 0; it < kNodeSteps; ++it) {
+            const bool searching = live && !done && !fallback && leaf == kNone;
+            if (__ballot(searching) == 0ull) break;
+            if (!live || done || fallback) continue;
+            if (!st.have) {
+                if (!wide_pop_next<kLdsStack, ANY>(st, ws)) {
+                    if (leaf == kNone) done = true;  // stack empty, nothing parked: the ray is finished
+                    continue;
                 }
-                ray = kNone;
             }
+            if (st.cur & kLeafBit) {
+                if (leaf == kNone) {  // park the first leaf, keep walking
+                    leaf = st.cur;
+                    st.have = false;
+                }
+                continue;  // a second leaf waits in st.cur
+            }
+            st.have = false;
+            if (wide_node<kLdsStack, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
+        }
+        // Leaf phase
+        if (leaf != kNone && !fallback && wide_leaf<ANY>(a.sv, st, leaf, a.tmin, a.tmax) == kStepDone) done = true;
+        if (live && !done && !fallback && !st.have && st.sp == 0) done = true;
+        if (done || fallback) {
+            HitOut h;
+            int rc = fallback ? kStepFallback : wide_finish<ANY>(a.sv, st, a.tmin, h);
+            if (rc == kStepFallback) {
+                uint32_t slot = atomicAdd(a.fb_count, 1u);
+                a.fb_list[slot] = ray;
+            } else {
+                write_out(a, ray, h);
+            }
+            ray = kNone;
         }
     }
 }

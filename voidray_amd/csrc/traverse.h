@@ -121,6 +121,7 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
 
 template <int LDS_STACK>
 VR_HD bool wide_push(WideState& st, const WideStack& ws, u32x2 e) {
+    static_assert((LDS_STACK & (LDS_STACK - 1)) == 0, "the LDS ring is indexed with a mask");
     if (st.sp - st.lo == LDS_STACK) {
         if (st.lo >= ws.spill_cap) return false;
         ws.spill[st.lo * ws.spill_stride] = ws.lds[(st.lo & (LDS_STACK - 1)) * ws.stride];
@@ -141,56 +142,66 @@ VR_HD u32x2 wide_pop(WideState& st, const WideStack& ws) {
     return ws.lds[(st.sp & (LDS_STACK - 1)) * ws.stride];
 }
 
-// One traversal step (one node or one leaf).  Returns kStepContinue,
-// kStepDone (result ready for wide_finish) or kStepFallback.
+// Pops the nearest pending entry that survives culling into st.cur; false
+// when the stack is empty.
 template <int LDS_STACK, bool ANY>
-VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
-    if (!st.have) {
-        for (;;) {
-            if (st.sp == 0) return kStepDone;
-            u32x2 e = wide_pop<LDS_STACK>(st, ws);
-            if (!ANY && u2f(e.y) > st.tbest * kCullSlack) continue;
-            st.cur = e.x;
-            break;
-        }
+VR_HD bool wide_pop_next(WideState& st, const WideStack& ws) {
+    while (st.sp > 0) {
+        u32x2 e = wide_pop<LDS_STACK>(st, ws);
+        if (!ANY && u2f(e.y) > st.tbest * kCullSlack) continue;
+        st.cur = e.x;
+        st.have = true;
+        return true;
     }
-    st.have = false;
+    return false;
+}
+
+// Tests the triangles of leaf `code`.  Returns kStepContinue, or kStepDone
+// when an any-hit query has its confirmed hit.
+template <bool ANY>
+VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmin, float tmax_any) {
     const vf3 o = st.o, d = st.d;
-    uint32_t cur = st.cur;
-    if (cur & kLeafBit) {
-        uint32_t first = cur & 0x07FFFFFFu;
-        uint32_t cnt = ((cur >> 27) & 7u) + 1u;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            vf3 p0, p1, p2;
-            load_tri_geom(sv.tris, first + k, p0, p1, p2);
-            MTResult r;
-            if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
-            if (ANY) {
-                if (r.t < tmax_any) {
-                    if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
-                        st.found = true;
-                        st.tbest = r.t;
-                        st.best_slot = first + k;
-                        st.bu = r.u;
-                        st.bv = r.v;
-                        return kStepDone;
-                    }
-                    st.unconfirmed = true;
+    uint32_t first = code & 0x07FFFFFFu;
+    uint32_t cnt = ((code >> 27) & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        vf3 p0, p1, p2;
+        load_tri_geom(sv.tris, first + k, p0, p1, p2);
+        MTResult r;
+        if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
+        if (ANY) {
+            if (r.t < tmax_any) {
+                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
+                    st.found = true;
+                    st.tbest = r.t;
+                    st.best_slot = first + k;
+                    st.bu = r.u;
+                    st.bv = r.v;
+                    return kStepDone;
                 }
-                continue;
+                st.unconfirmed = true;
             }
-            if (r.t > st.tbest) continue;
-            uint32_t rank = tri_word(sv.tris, first + k, 10);
-            if (r.t < st.tbest || rank > st.best_rank) {
-                st.tbest = r.t;
-                st.best_rank = rank;
-                st.best_slot = first + k;
-                st.bu = r.u;
-                st.bv = r.v;
-            }
+            continue;
         }
-        return kStepContinue;
+        if (r.t > st.tbest) continue;
+        uint32_t rank = tri_word(sv.tris, first + k, 10);
+        if (r.t < st.tbest || rank > st.best_rank) {
+            st.tbest = r.t;
+            st.best_rank = rank;
+            st.best_slot = first + k;
+            st.bu = r.u;
+            st.bv = r.v;
+        }
     }
+    return kStepContinue;
+}
+
+// Visits inner node `cur`: children in near-to-far order, the nearest becomes
+// st.cur (st.have), the others are pushed.  Returns kStepContinue, or
+// kStepFallback when the stack is exhausted (the exact engine answers).
+template <int LDS_STACK, bool ANY>
+VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t cur, float tmin,
+                    float tmax_any) {
+    const vf3 o = st.o, d = st.d;
     const u32x4* qp = (const u32x4*)(sv.qnodes + cur);
     u32x4 w0 = qp[0], w1 = qp[1], w2 = qp[2], w3 = qp[3];
     uint32_t e = w0.w;
@@ -247,7 +258,21 @@ VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, flo
         if (tn[c] == kInf) continue;
         ok = ok && wide_push<LDS_STACK>(st, ws, u32x2{cid[c], f2u(tn[c])});
     }
-    return ok ? kStepContinue : kStepFallback;  // stack exhausted: the exact engine answers
+    return ok ? kStepContinue : kStepFallback;
+}
+
+// One traversal step (one node or one leaf).  Returns kStepContinue,
+// kStepDone (result ready for wide_finish) or kStepFallback.
+template <int LDS_STACK, bool ANY>
+VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
+    if (!st.have && !wide_pop_next<LDS_STACK, ANY>(st, ws)) return kStepDone;
+    st.have = false;
+    const uint32_t cur = st.cur;
+    if (cur & kLeafBit) {
+        int rc = wide_leaf<ANY>(sv, st, cur, tmin, tmax_any);
+        return rc == kStepDone ? kStepDone : kStepContinue;
+    }
+    return wide_node<LDS_STACK, ANY>(sv, st, ws, cur, tmin, tmax_any);
 }
 
 // Result of a finished query (after kStepDone).  Returns kStepDone or
