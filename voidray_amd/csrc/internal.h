@@ -105,6 +105,8 @@ struct SceneView {
     const float* vnrm;
     const float* vuv;
     const uint32_t* prim_surface; // global prim -> surface
+    const int32_t* ref_parent;    // per ref node: parent ref node, -1 at a mesh root
+    const int32_t* prim_anc;      // global prim -> parent of its leaf in the mesh tree (-1: none)
     const GpuMaterial* materials;
     const GpuTexture* textures;
     const uint8_t* tex_data;
@@ -143,8 +145,8 @@ struct vr_scene {
     vr_scene_stats stats{};
     // scratch for traces (grown on demand)
     uint32_t* fallback_list = nullptr;
-    uint32_t* counters = nullptr;  // [0] fallback count, [1..] misc
-    uint32_t* host_counters = nullptr;  // pinned mirror of counters, read after a sync
+    uint32_t* counters = nullptr;       // trace counter block (64 words; layout in trace.hip)
+    uint32_t* host_counters = nullptr;  // pinned: [0] fallbacks, [1] overflows of the last vr_trace*
     uint32_t fallback_cap = 0;
     u32x2* spill = nullptr;             // per-lane spill stacks of the persistent traversal
     uint32_t spill_lanes = 0;
@@ -164,6 +166,7 @@ struct HostScene {
     std::vector<TriRec> tris;
     std::vector<RefNode> ref_nodes, top_nodes;
     std::vector<uint32_t> top_path, mesh_rank, mesh_idx, prim_surface;
+    std::vector<int32_t> ref_parent, prim_anc, mesh_leaf_parent;
     std::vector<MeshInfo> meshes;
     std::vector<SurfaceInfo> surfaces;
     std::vector<float> face_n, vpos, vnrm, vuv, env, env_marg, env_cond, env_pdf;

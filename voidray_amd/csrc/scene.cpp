@@ -68,6 +68,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     std::vector<float>&vpos = H.vpos, &vnrm = H.vnrm, &vuv = H.vuv, &face_n = H.face_n;
     std::vector<uint32_t>&midx = H.mesh_idx, &mrank = H.mesh_rank;
     std::vector<RefNode>& ref_nodes = H.ref_nodes;
+    std::vector<int32_t>&ref_parent = H.ref_parent, &leaf_parent = H.mesh_leaf_parent;
     std::vector<Box3> mesh_bounds(M);
     uint64_t tv = 0, tt = 0;
     for (uint32_t m = 0; m < M; ++m) {
@@ -128,12 +129,28 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
                 if (nd.obj < 0) nd.right += off;
             mi.root = off;
             ref_nodes.insert(ref_nodes.end(), tree.begin(), tree.end());
+            // parents (depth-first layout: left child = k + 1) and each
+            // triangle's leaf parent, for the exact ancestor walk
+            ref_parent.resize(ref_nodes.size(), -1);
+            leaf_parent.resize(midx.size() / 3, -1);
+            for (size_t k = 0; k < tree.size(); ++k) {
+                const RefNode& nd = tree[k];
+                int32_t g = off + (int32_t)k;
+                if (nd.obj < 0) {
+                    ref_parent[(size_t)g + 1] = g;
+                    ref_parent[(size_t)nd.right] = g;
+                }
+            }
+            for (size_t k = 0; k < tree.size(); ++k)
+                if (tree[k].obj >= 0) leaf_parent[mi.tri_base + (uint32_t)tree[k].obj] = ref_parent[off + k];
             mrank.insert(mrank.end(), rank.begin(), rank.end());
         } else {
             // brute-force loop keeps the earlier triangle on equal t (mesh.rs:131)
             for (uint32_t t = 0; t < src.num_triangles; ++t) mrank.push_back(src.num_triangles - 1 - t);
         }
     }
+    ref_parent.resize(ref_nodes.size(), -1);
+    leaf_parent.resize(midx.size() / 3, -1);
 
     // ---- top-level tree over surfaces (scene.rs:72-92, 176) ---------------
     uint32_t S = d->num_surfaces;
@@ -191,6 +208,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     std::vector<TriRec> recs(T);
     std::vector<uint32_t>& prim_surface = H.prim_surface;
     prim_surface.assign(T, 0);
+    std::vector<int32_t>& prim_anc = H.prim_anc;
+    prim_anc.assign(T, -1);
     for (uint32_t i = 0; i < S; ++i) {
         const MeshInfo& mi = minfo[sinfo[i].mesh];
         for (uint32_t t = 0; t < mi.num_tris; ++t) {
@@ -205,6 +224,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             r.rank = sinfo[i].rank_base + mrank[mi.tri_base + t];
             r.flags = (mi.root >= 0 ? 1u : 0u) | (i << 1);
             prim_surface[sinfo[i].prim_base + t] = i;
+            prim_anc[sinfo[i].prim_base + t] = leaf_parent[mi.tri_base + t];
         }
     }
     std::vector<QNode>& qn = H.qnodes;
@@ -286,6 +306,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.vnrm = vnrm.data();
     v.vuv = vuv.data();
     v.prim_surface = prim_surface.data();
+    v.ref_parent = ref_parent.data();
+    v.prim_anc = prim_anc.data();
     v.materials = mats.data();
     v.textures = texs.data();
     v.tex_data = texdata.data();
@@ -331,6 +353,8 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.vnrm, &v.vnrm);
     rc |= upload(s, H.vuv, &v.vuv);
     rc |= upload(s, H.prim_surface, &v.prim_surface);
+    rc |= upload(s, H.ref_parent, &v.ref_parent);
+    rc |= upload(s, H.prim_anc, &v.prim_anc);
     rc |= upload(s, H.materials, &v.materials);
     rc |= upload(s, H.textures, &v.textures);
     rc |= upload(s, H.tex_data, &v.tex_data);

@@ -6,7 +6,8 @@
 namespace vr {
 
 constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
-constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave
+constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave, 5 waves/SIMD
+constexpr int kLdsStackAny = 8;  // any-hit: 4 KB per wave, so registers (7 waves/SIMD) bound occupancy
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
 constexpr int kRefill = 16;      // idle lanes that trigger a wave-wide refill (closest hit)
 constexpr int kRefillAny = 32;   // the same for any-hit queries (shorter rays: refill less often)
@@ -49,7 +50,8 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
 // (dynamic ray fetch), so long incoherent rays do not leave the wave idle.
 template <bool ANY>
 __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
-    __shared__ u32x2 lds[kLdsStack * kWave];
+    constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
+    __shared__ u32x2 lds[LDS * kWave];
     const int lane = threadIdx.x;
     const uint32_t glane = blockIdx.x * kWave + lane;
     const WideStack ws{lds + lane, kWave, a.spill + glane, (int)a.spill_lanes, kSpillCap};
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
             if (__ballot(searching) == 0ull) break;
             if (!live || done || fallback) continue;
             if (!st.have) {
-                if (!wide_pop_next<kLdsStack, ANY>(st, ws)) {
+                if (!wide_pop_next<LDS, ANY>(st, ws)) {
                     if (leaf == kNone) done = true;  // stack empty, nothing parked: the ray is finished
                     continue;
                 }
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
                 continue;  // a second leaf waits in st.cur
             }
             st.have = false;
-            if (wide_node<kLdsStack, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
+            if (wide_node<LDS, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
         }
         // Leaf phase
         if (leaf != kNone && !fallback && wide_leaf<ANY>(a.sv, st, leaf, a.tmin, a.tmax) == kStepDone) done = true;
@@ -125,18 +127,32 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     }
 }
 
-// Exact engine over a ray list (fallbacks) or over all rays (list == nullptr).
-__global__ __launch_bounds__(64) void k_trace_exact(TraceArgs a, const uint32_t* list, const uint32_t* count) {
-    uint32_t n = list ? *count : (a.count ? *a.count : a.n);
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        uint32_t i = list ? list[j] : (a.queue ? a.queue[j] : j);
-        vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
-        vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
-        HitOut h;
-        trace_exact(a.sv, o, d, a.tmin, h);
-        if (a.any && h.prim != 0xFFFFFFFFu && !(h.t < a.tmax)) h.prim = 0xFFFFFFFFu;
-        write_out(a, i, h);
-    }
+// Words of a scene's trace counter block.  The wide kernel appends to the
+// fallback list and fetches rays with kCtrNext; the cooperative exact kernel
+// appends overflows, counts finished blocks, and its last block publishes the
+// launch's counts and clears the working words, so every launch starts from
+// zero without a reset launch (or a memset node in a captured graph).
+enum : int {
+    kCtrFallbacks = 0,      // rays handed from the wide engine to the exact engine
+    kCtrNext = 1,           // dynamic ray fetch of the persistent kernel
+    kCtrOverflow = 2,       // rays whose frontier outgrew the cooperative LDS stack
+    kCtrBlocksDone = 3,     // finished blocks of the cooperative exact kernel
+    kCtrLastFallbacks = 8,  // published: fallbacks of the last launch
+    kCtrLastOverflow = 9,   // published: overflows of the last launch
+};
+
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Serial exact engine for one ray, with its stacks in the caller's LDS.
+__device__ __forceinline__ void exact_serial(const TraceArgs& a, uint32_t i, int32_t* tstack, int32_t* mstack, int stride) {
+    vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
+    vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
+    HitOut h;
+    trace_exact_stk(a.sv, o, d, a.tmin, h, tstack, mstack, stride);
+    if (a.any && h.prim != 0xFFFFFFFFu && !(h.t < a.tmax)) h.prim = 0xFFFFFFFFu;
+    write_out(a, i, h);
 }
 
 // Wave-cooperative exact engine: one wave per listed ray.  The reference
@@ -145,16 +161,30 @@ __global__ __launch_bounds__(64) void k_trace_exact(TraceArgs a, const uint32_t*
 // pending nodes at once from an LDS stack, tests them in parallel and pushes
 // the children of passing nodes.  The winner is the lexicographic minimum of
 // (t, -rank) over the wave, identical to trace_exact.  A ray whose frontier
-// outgrows the LDS stack is re-queued for the serial exact kernel.
+// outgrows the LDS stack is listed for the serial engine, which the last block
+// to finish runs over the list, one ray per lane.  `list` == nullptr: every
+// ray (or every queued ray) is traced; otherwise the wide engine's fallbacks.
 constexpr int kCoopStack = 6144;
+static_assert(kCoopStack * 2 >= 2 * 64 * kWave, "the serial stacks reuse the cooperative stack");
 enum : uint32_t { kCoopTop = 0u, kCoopMesh = 1u, kCoopTri = 2u };
 
-__global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const uint32_t* list, const uint32_t* count,
-                                                            uint32_t* serial_list, uint32_t* serial_count) {
+__global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const uint32_t* list, uint32_t* ctr,
+                                                            uint32_t* serial_list, uint64_t* accum) {
     __shared__ u32x2 stk[kCoopStack];
     const int lane = threadIdx.x;
     const SceneView& sv = a.sv;
-    uint32_t n = list ? *count : (a.count ? *a.count : a.n);
+    // (agent-scope loads: a select between a device count and the by-value
+    // argument would otherwise be folded into one load through a pointer to a
+    // private copy of the arguments)
+    uint32_t n = a.n;
+    if (list) n = load_agent(&ctr[kCtrFallbacks]);
+    else if (a.count) n = load_agent(a.count);
+    // Blocks without a ray leave at once; the others (at least block 0) take
+    // part in the exit count.  A block that leaves may read the count after
+    // the reset, but then it reads 0 and leaves all the same: the reset only
+    // happens once every taking-part block has read the count.
+    const uint32_t parts = n < gridDim.x ? (n > 1u ? n : 1u) : gridDim.x;
+    if (blockIdx.x >= parts) return;
     for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
         uint32_t ray = list ? list[j] : (a.queue ? a.queue[j] : j);
         vf3 o = mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]);
@@ -172,9 +202,11 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
             if (act) e = stk[sp - take + lane];
             __syncthreads();
             sp -= take;
-            // children to push (up to 4: two tree children or up to four small-mesh triangles)
-            u32x2 out[4];
+            // entries to push: two tree children (e0 = right, e1 = left), one
+            // mesh root, or a run of np small-mesh triangles (e0.x + k)
+            u32x2 e0 = u32x2{0u, 0u}, e1 = u32x2{0u, 0u};
             int np = 0;
+            bool run = false;
             if (act) {
                 uint32_t kind = e.x >> 30, idx = e.x & 0x3FFFFFFFu, surf = e.y;
                 if (kind == kCoopTop) {
@@ -183,15 +215,19 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         uint32_t s2 = (uint32_t)nd.obj;
                         MeshInfo mi = sv.meshes[sv.surfaces[s2].mesh];
                         if (mi.root >= 0) {
-                            out[np++] = u32x2{(kCoopMesh << 30) | (uint32_t)mi.root, s2};
+                            e0 = u32x2{(kCoopMesh << 30) | (uint32_t)mi.root, s2};
+                            np = 1;
                         } else {
-                            for (uint32_t k = 0; k < mi.num_tris && k < 4; ++k) out[np++] = u32x2{(kCoopTri << 30) | k, s2};
+                            e0 = u32x2{kCoopTri << 30, s2};
+                            np = (int)(mi.num_tris < 4u ? mi.num_tris : 4u);
+                            run = true;
                         }
                     } else {
                         Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
                         if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
-                            out[np++] = u32x2{(kCoopTop << 30) | (uint32_t)nd.right, 0u};
-                            out[np++] = u32x2{(kCoopTop << 30) | (idx + 1), 0u};
+                            e0 = u32x2{(kCoopTop << 30) | (uint32_t)nd.right, 0u};
+                            e1 = u32x2{(kCoopTop << 30) | (idx + 1), 0u};
+                            np = 2;
                         }
                     }
                 } else {
@@ -205,8 +241,9 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         } else {
                             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
                             if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
-                                out[np++] = u32x2{(kCoopMesh << 30) | (uint32_t)nd.right, surf};
-                                out[np++] = u32x2{(kCoopMesh << 30) | (idx + 1), surf};
+                                e0 = u32x2{(kCoopMesh << 30) | (uint32_t)nd.right, surf};
+                                e1 = u32x2{(kCoopMesh << 30) | (idx + 1), surf};
+                                np = 2;
                             }
                         }
                     } else {
@@ -245,14 +282,15 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                 overflow = true;
                 break;
             }
-            for (int k = 0; k < np; ++k) stk[base + k] = out[k];
+            for (int k = 0; k < np; ++k)
+                stk[base + k] = run ? u32x2{e0.x + (uint32_t)k, e0.y} : (k == 0 ? e0 : e1);
             sp += total;
             __syncthreads();
         }
         __syncthreads();
         if (overflow) {
             if (lane == 0) {
-                uint32_t slot = atomicAdd(serial_count, 1u);
+                uint32_t slot = atomicAdd(&ctr[kCtrOverflow], 1u);
                 serial_list[slot] = ray;
             }
             continue;
@@ -284,15 +322,30 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
             write_out(a, ray, h);
         }
     }
-}
-
-__global__ void k_add_u64(uint64_t* acc, const uint32_t* v) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *v;
-}
-
-__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    // count this block out; the last one runs the overflow list and resets
+    // (one same-address atomic per taking-part block, not per block)
+    __threadfence();
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(&ctr[kCtrBlocksDone], 1u);
+    ticket = __shfl(ticket, 0);
+    if (ticket != parts - 1) return;
+    __threadfence();
+    const uint32_t ns = load_agent(&ctr[kCtrOverflow]);
+    if (ns) {  // the cooperative stack is free now: 64 + 64 stack entries per lane
+        int32_t* lds32 = (int32_t*)stk;
+        for (uint32_t k = lane; k < ns; k += kWave)
+            exact_serial(a, load_agent(&serial_list[k]), lds32 + lane, lds32 + 64 * kWave + lane, kWave);
+    }
+    if (lane == 0) {
+        uint32_t fb = list ? load_agent(&ctr[kCtrFallbacks]) : 0u;
+        if (accum) *accum += fb;
+        ctr[kCtrLastFallbacks] = fb;
+        ctr[kCtrLastOverflow] = ns;
+        ctr[kCtrFallbacks] = 0u;
+        ctr[kCtrNext] = 0u;
+        ctr[kCtrOverflow] = 0u;
+        ctr[kCtrBlocksDone] = 0u;
+    }
 }
 
 static int persist_grid(vr_scene* s, bool any, uint32_t& blocks) {
@@ -325,6 +378,8 @@ int ensure_trace_scratch(vr_scene* s, uint32_t n) {
         VR_HIP_CHECK(hipMalloc((void**)&s->counters, 64 * sizeof(uint32_t)));
         VR_HIP_CHECK(hipHostMalloc((void**)&s->host_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
         for (int k = 0; k < 64; ++k) s->host_counters[k] = 0;
+        // starts at zero; from then on every launch leaves it at zero
+        VR_HIP_CHECK(hipMemcpy(s->counters, s->host_counters, 64 * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     // two lists of n entries: wide-engine fallbacks, then the serial-exact overflow list
     if (s->fallback_cap < 2 * n) {
@@ -343,8 +398,8 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
     if (rc) return rc;
     TraceArgs a{s->view, j.o, j.d, j.n_max, j.tmin, j.any ? 1 : 0, j.prim, j.t, j.u, j.v, j.hit,
                 s->fallback_list, s->counters, s->counters + 1, s->spill, s->spill_lanes, j.queue, j.count, j.tmax};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, s->counters, 4u);
     uint32_t* serial_list = s->fallback_list + s->fallback_cap / 2;
+    uint64_t* accum = j.engine == VR_ENGINE_WIDE ? j.fallback_accum : nullptr;
     if (j.engine == VR_ENGINE_WIDE) {
         uint32_t blocks = 0;
         rc = persist_grid(s, j.any, blocks);
@@ -356,19 +411,14 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
         else
             hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
         if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[1], stream));
-        // fallbacks: one wave per ray, grid reads the device-side count (no host sync);
-        // rays whose frontier outgrows LDS go to the serial exact kernel
-        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)s->fallback_list,
-                           (const uint32_t*)s->counters, serial_list, s->counters + 2);
+        // fallbacks: one wave per ray; the grid reads the device-side count (no host sync)
+        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a,
+                           (const uint32_t*)s->fallback_list, s->counters, serial_list, accum);
     } else {
         uint32_t blocks = std::min<uint32_t>(j.n_max, 65535u);
         hipLaunchKernelGGL(k_trace_exact_wave, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr,
-                           (const uint32_t*)nullptr, serial_list, s->counters + 2);
+                           s->counters, serial_list, accum);
     }
-    hipLaunchKernelGGL(k_trace_exact, dim3(256), dim3(64), 0, stream, a, (const uint32_t*)serial_list,
-                       (const uint32_t*)(s->counters + 2));
-    if (j.fallback_accum && j.engine == VR_ENGINE_WIDE)
-        hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, stream, j.fallback_accum, (const uint32_t*)s->counters);
     VR_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -402,7 +452,8 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     s->trace_timed = j.timed;
     int rc = launch_trace(s, j, stream);
     if (rc) return rc;
-    VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters + kCtrLastFallbacks, 2 * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, stream));
     return 0;
 }
 

@@ -9,12 +9,13 @@
 //            slack on the slab exit, a relative slack on the cull distance), so
 //            every triangle the reference could return is reached.  Equal-t
 //            hits are resolved by the reference's in-order rank.  The winner is
-//            then confirmed: its hit point must lie strictly inside the
-//            eps-expanded triangle box by a margin that dominates the rounding
-//            of the reference's slab test (so every reference ancestor box
-//            passes), and its top-level ancestors must pass the exact reference
-//            slab test.  Unconfirmed winners and stack overflows go to the
-//            exact engine.
+//            then confirmed to be a triangle the reference reaches: its
+//            top-level ancestors must pass the exact reference slab test, and
+//            so must its mesh-tree ancestors — taken as passed when the hit
+//            point lies strictly inside the eps-expanded triangle box by a
+//            margin that dominates the slab test's rounding, walked exactly
+//            otherwise.  Winners the reference does not reach and stack
+//            overflows go to the exact engine.
 //  * exact — the flattened reference trees walked exactly as BvhNode::hit
 //            (core/bvh.rs:132-181): both children, unshrunk t_max, no box test
 //            at leaves, brute force for meshes of <= 4 triangles
@@ -49,9 +50,27 @@ VR_HD void load_tri_geom(const TriRec* tris, uint32_t slot, vf3& p0, vf3& p1, vf
 }
 VR_HD uint32_t tri_word(const TriRec* tris, uint32_t slot, int w) { return ((const uint32_t*)(tris + slot))[w]; }
 
+// Exact membership of a triangle in the reference's candidate set within its
+// mesh: BvhNode::hit (core/bvh.rs:132-181) descends into a child only after
+// the parent's box passes the slab test, and tests leaves without a box, so
+// the triangle is reached iff every inner ancestor of its leaf passes
+// AABB::hit with the unshrunk t_max.  Walks leaf parent -> mesh root with the
+// reference's own slab arithmetic, so the answer is the reference's.
+VR_HD bool mesh_ancestors_pass(const SceneView& sv, vf3 o, vf3 d, float tmin, uint32_t prim) {
+    int32_t nd = sv.prim_anc[prim];
+    while (nd >= 0) {
+        const RefNode& n = sv.ref_nodes[nd];
+        int32_t up = sv.ref_parent[nd];
+        Box3 b{mk3(n.lo[0], n.lo[1], n.lo[2]), mk3(n.hi[0], n.hi[1], n.hi[2])};
+        if (!ref_slab_hit(b, o, d, tmin, kInf)) return false;
+        nd = up;
+    }
+    return true;
+}
+
 // Confirms that the reference would consider this hit (see header comment).
 VR_HD bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t, vf3 p0, vf3 p1,
-                                               vf3 p2, uint32_t flags) {
+                                               vf3 p2, uint32_t flags, uint32_t prim) {
     uint32_t surf = flags >> 1;
     if (sv.num_surfaces > 1) {
         SurfaceInfo si = sv.surfaces[surf];
@@ -62,14 +81,19 @@ VR_HD bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t
         }
     }
     if (flags & 1u) {
+        // fast path: a hit point inside the eps-expanded triangle box by a
+        // margin that dominates the slab test's rounding passes every
+        // ancestor box; otherwise walk the ancestors exactly
         vf3 P = o + d * t;  // Ray::at (ray.rs:19-21)
         Box3 bb = tri_box_eps(p0, p1, p2);
+        bool inside = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float pa = comp(P, a);
             float delta = kRobustRel * (t + fabsf(pa) + fabsf(comp(o, a)));
-            if (!(pa - comp(bb.lo, a) > delta && comp(bb.hi, a) - pa > delta)) return false;
+            inside = inside && pa - comp(bb.lo, a) > delta && comp(bb.hi, a) - pa > delta;
         }
+        if (!inside) return mesh_ancestors_pass(sv, o, d, tmin, prim);
     }
     return true;
 }
@@ -170,7 +194,8 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
         if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
         if (ANY) {
             if (r.t < tmax_any) {
-                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11))) {
+                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11),
+                                   tri_word(sv.tris, first + k, 9))) {
                     st.found = true;
                     st.tbest = r.t;
                     st.best_slot = first + k;
@@ -296,7 +321,8 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
     if (st.best_slot == 0xFFFFFFFFu) return kStepDone;
     vf3 p0, p1, p2;
     load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
-    if (!confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11)))
+    if (!confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
+                        tri_word(sv.tris, st.best_slot, 9)))
         return kStepFallback;
     out.t = st.tbest;
     out.u = st.bu;
@@ -321,25 +347,27 @@ VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_a
     return wide_finish<ANY>(sv, st, tmin, out);
 }
 
-// Exact reference engine for one ray (closest hit).
-__host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out) {
+// Exact reference engine for one ray (closest hit).  The two traversal
+// stacks (64 entries each, element k at [k * stride]) are the caller's: local
+// arrays, or a lane's slice of LDS.
+__host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out,
+                                                int32_t* tstack, int32_t* mstack, int stride) {
     out.prim = 0xFFFFFFFFu;
     out.t = out.u = out.v = 0.f;
     if (!sv.top_root_valid) return;
     bool have = false;
     float bt = 0.f;
     uint32_t brank = 0;
-    int32_t tstack[64];
     int tsp = 0;
-    tstack[tsp++] = 0;
+    tstack[tsp++ * stride] = 0;
     while (tsp) {
-        int32_t ni = tstack[--tsp];
+        int32_t ni = tstack[--tsp * stride];
         RefNode tn = sv.top_nodes[ni];
         if (tn.obj < 0) {
             Box3 b{mk3(tn.lo[0], tn.lo[1], tn.lo[2]), mk3(tn.hi[0], tn.hi[1], tn.hi[2])};
             if (ref_slab_hit(b, o, d, tmin, kInf)) {
-                tstack[tsp++] = tn.right;
-                tstack[tsp++] = ni + 1;
+                tstack[tsp++ * stride] = tn.right;
+                tstack[tsp++ * stride] = ni + 1;
             }
             continue;
         }
@@ -369,11 +397,10 @@ __host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, f
             for (uint32_t k = 0; k < mi.num_tris; ++k) test(k);
             continue;
         }
-        int32_t mstack[64];
         int msp = 0;
-        mstack[msp++] = mi.root;
+        mstack[msp++ * stride] = mi.root;
         while (msp) {
-            int32_t mj = mstack[--msp];
+            int32_t mj = mstack[--msp * stride];
             RefNode nd = sv.ref_nodes[mj];
             if (nd.obj >= 0) {
                 test((uint32_t)nd.obj);
@@ -381,10 +408,15 @@ __host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, f
             }
             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
             if (!ref_slab_hit(b, o, d, tmin, kInf)) continue;
-            mstack[msp++] = nd.right;
-            mstack[msp++] = mj + 1;
+            mstack[msp++ * stride] = nd.right;
+            mstack[msp++ * stride] = mj + 1;
         }
     }
+}
+
+__host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out) {
+    int32_t tstack[64], mstack[64];
+    trace_exact_stk(sv, o, d, tmin, out, tstack, mstack, 1);
 }
 
 }  // namespace vr
