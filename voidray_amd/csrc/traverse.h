@@ -237,21 +237,21 @@ VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uin
     float by = (u2f(w0.y) - o.y) * st.idy;
     float bz = (u2f(w0.z) - o.z) * st.idz;
     uint32_t ch[4] = {w1.x, w1.y, w1.z, w1.w};
-    uint32_t qlx = w2.x, qly = w2.y, qlz = w2.z, qhx = w2.w, qhy = w3.x, qhz = w3.y;
+    // Near and far planes chosen once per node from the ray's direction signs
+    // (all four children's bytes at once): with s > 0, fma(lo, s, b) <=
+    // fma(hi, s, b), so this equals taking min/max per child and axis.
+    const bool negx = st.idx < 0.f, negy = st.idy < 0.f, negz = st.idz < 0.f;
+    uint32_t qnx = negx ? w2.w : w2.x, qfx = negx ? w2.x : w2.w;
+    uint32_t qny = negy ? w3.x : w2.y, qfy = negy ? w2.y : w3.x;
+    uint32_t qnz = negz ? w3.y : w2.z, qfz = negz ? w2.z : w3.y;
     float tcull = ANY ? tmax_any : st.tbest * kCullSlack;
     float tn[4];
     uint32_t cid[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        float lx = (float)((qlx >> (8 * c)) & 0xFFu), hx = (float)((qhx >> (8 * c)) & 0xFFu);
-        float ly = (float)((qly >> (8 * c)) & 0xFFu), hy = (float)((qhy >> (8 * c)) & 0xFFu);
-        float lz = (float)((qlz >> (8 * c)) & 0xFFu), hz = (float)((qhz >> (8 * c)) & 0xFFu);
-        float t0x = fmaf(lx, sx, bx), t1x = fmaf(hx, sx, bx);
-        float t0y = fmaf(ly, sy, by), t1y = fmaf(hy, sy, by);
-        float t0z = fmaf(lz, sz, bz), t1z = fmaf(hz, sz, bz);
-        float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
-        float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
-        float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
+        float nx = fmaf((float)((qnx >> (8 * c)) & 0xFFu), sx, bx), fx = fmaf((float)((qfx >> (8 * c)) & 0xFFu), sx, bx);
+        float ny = fmaf((float)((qny >> (8 * c)) & 0xFFu), sy, by), fy = fmaf((float)((qfy >> (8 * c)) & 0xFFu), sy, by);
+        float nz = fmaf((float)((qnz >> (8 * c)) & 0xFFu), sz, bz), fz = fmaf((float)((qfz >> (8 * c)) & 0xFFu), sz, bz);
         float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
         float tfar = fminf(fminf(fx, fy), fminf(fz, tcull));
         bool hit = ch[c] != kInvalidChild && tnear <= tfar * kSlabSlack;
