@@ -289,6 +289,168 @@ struct SahBuilder {
     }
 };
 
+// Outward padding of a triangle (or piece) box by 8 ulps of the coordinate
+// magnitude: absorbs the rounding of the slab test's subtraction.
+static void pad_box(BBox& b) {
+    for (int a = 0; a < 3; ++a) {
+        float m = std::max(std::fabs(b.lo[a]), std::fabs(b.hi[a]));
+        float pad = std::max(m * 1.1920929e-7f * 8.f, 1e-30f);
+        b.lo[a] -= pad;
+        b.hi[a] += pad;
+    }
+}
+
+// ---- early split clipping (Ernst & Greiner 2007) ---------------------------
+// A long, thin triangle lying diagonally has a box far larger than itself, and
+// every ray through that empty space visits its leaf.  Such a triangle is
+// referenced from several leaves instead: its box is cut at the midpoint of
+// the longest axis, the triangle is clipped to each half (in double), and each
+// piece gets the box of its clipped polygon, recursively while a cut removes
+// enough box area.  The pieces' boxes cover the whole triangle (each point of
+// it lies in one half, and each piece box is rounded outward and padded), so a
+// ray that hits the triangle reaches at least one leaf holding it; duplicate
+// records of a triangle give the same t and rank, so the result is unchanged.
+struct DV3 {
+    double v[3];
+};
+
+static int clip_half(const DV3* in, int n, int axis, double plane, bool keep_below, DV3* out) {
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const DV3& a = in[i];
+        const DV3& b = in[(i + 1) % n];
+        double da = keep_below ? plane - a.v[axis] : a.v[axis] - plane;
+        double db = keep_below ? plane - b.v[axis] : b.v[axis] - plane;
+        if (da >= 0) out[m++] = a;
+        if ((da >= 0) != (db >= 0)) {
+            double t = da / (da - db);
+            DV3 p;
+            for (int k = 0; k < 3; ++k) p.v[k] = a.v[k] + t * (b.v[k] - a.v[k]);
+            p.v[axis] = plane;
+            out[m++] = p;
+        }
+    }
+    return m;
+}
+
+static double dbox_area(const double lo[3], const double hi[3]) {
+    double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+static void poly_bounds(const DV3* p, int n, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], p[i].v[a]);
+            hi[a] = std::max(hi[a], p[i].v[a]);
+        }
+}
+
+constexpr int kSplitDepth = 2;          // at most 4 pieces per triangle
+constexpr double kSplitKeep = 0.7;      // cut only if the halves keep < 70% of the box area
+constexpr double kSplitRatio = 14.0;    // split only triangles whose box area exceeds 14x their area
+
+static double tri_area(const DV3* p) {
+    double e1[3], e2[3];
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = p[1].v[a] - p[0].v[a];
+        e2[a] = p[2].v[a] - p[0].v[a];
+    }
+    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+static void split_piece(const DV3* poly, int n, int depth, const BBox& tri_box, std::vector<BBox>& out) {
+    double lo[3], hi[3];
+    poly_bounds(poly, n, lo, hi);
+    // An axis-aligned right triangle has a box of 4x its area and a tilted
+    // grid triangle (a heightfield's) about 6-12x: those are left whole.
+    bool emit = depth >= kSplitDepth || (depth == 0 && !(dbox_area(lo, hi) > kSplitRatio * tri_area(poly)));
+    DV3 pl[9], pr[9];
+    int nl = 0, nr = 0;
+    if (!emit) {
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+        double mid = 0.5 * (lo[axis] + hi[axis]);
+        nl = clip_half(poly, n, axis, mid, true, pl);
+        nr = clip_half(poly, n, axis, mid, false, pr);
+        if (nl < 3 || nr < 3) {
+            emit = true;
+        } else {
+            double ll[3], lh[3], rl[3], rh[3];
+            poly_bounds(pl, nl, ll, lh);
+            poly_bounds(pr, nr, rl, rh);
+            if (dbox_area(ll, lh) + dbox_area(rl, rh) > kSplitKeep * dbox_area(lo, hi)) emit = true;
+        }
+    }
+    if (emit) {
+        BBox b;
+        for (int a = 0; a < 3; ++a) {
+            float fl = (float)lo[a], fh = (float)hi[a];
+            if ((double)fl > lo[a]) fl = std::nextafter(fl, -INFINITY);
+            if ((double)fh < hi[a]) fh = std::nextafter(fh, INFINITY);
+            b.lo[a] = fl;
+            b.hi[a] = fh;
+        }
+        pad_box(b);
+        for (int a = 0; a < 3; ++a) {  // never larger than the whole triangle's box
+            b.lo[a] = std::max(b.lo[a], tri_box.lo[a]);
+            b.hi[a] = std::min(b.hi[a], tri_box.hi[a]);
+        }
+        out.push_back(b);
+        return;
+    }
+    split_piece(pl, nl, depth + 1, tri_box, out);
+    split_piece(pr, nr, depth + 1, tri_box, out);
+}
+
+static void split_references(const std::vector<TriRec>& tris, const std::vector<BBox>& pb, std::vector<BBox>& rb,
+                             std::vector<uint32_t>& rtri) {
+    const size_t n = tris.size();
+    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    size_t chunk = (n + hw - 1) / hw;
+    std::vector<std::vector<BBox>> boxes(hw);
+    std::vector<std::vector<uint32_t>> owners(hw);
+    std::vector<std::future<void>> jobs;
+    for (unsigned w = 0; w < hw; ++w) {
+        jobs.push_back(std::async(std::launch::async, [&, w] {
+            size_t b0 = w * chunk, b1 = std::min(n, b0 + chunk);
+            std::vector<BBox> tmp;
+            for (size_t i = b0; i < b1; ++i) {
+                DV3 poly[3];
+                for (int a = 0; a < 3; ++a) {
+                    poly[0].v[a] = tris[i].p0[a];
+                    poly[1].v[a] = tris[i].p1[a];
+                    poly[2].v[a] = tris[i].p2[a];
+                }
+                tmp.clear();
+                split_piece(poly, 3, 0, pb[i], tmp);
+                if (tmp.size() <= 1) {
+                    boxes[w].push_back(pb[i]);
+                    owners[w].push_back((uint32_t)i);
+                } else {
+                    for (auto& b : tmp) {
+                        boxes[w].push_back(b);
+                        owners[w].push_back((uint32_t)i);
+                    }
+                }
+            }
+        }));
+    }
+    for (auto& j : jobs) j.get();
+    rb.clear();
+    rtri.clear();
+    for (unsigned w = 0; w < hw; ++w) {
+        rb.insert(rb.end(), boxes[w].begin(), boxes[w].end());
+        rtri.insert(rtri.end(), owners[w].begin(), owners[w].end());
+    }
+}
+
 // Per-axis power-of-two grid for a node: smallest e with 255 * 2^e >= extent.
 static int pick_exponent(float lo, float hi) {
     double ext = (double)hi - (double)lo;
@@ -319,25 +481,29 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
         b.grow(tris_in[i].p0);
         b.grow(tris_in[i].p1);
         b.grow(tris_in[i].p2);
-        for (int a = 0; a < 3; ++a) {
-            float m = std::max(std::fabs(b.lo[a]), std::fabs(b.hi[a]));
-            float pad = std::max(m * 1.1920929e-7f * 8.f, 1e-30f);  // 8 ulps of the magnitude
-            b.lo[a] -= pad;
-            b.hi[a] += pad;
-        }
+        pad_box(b);
         pb[i] = b;
     }
-    SahBuilder sb(pb);
+    // references: a triangle, or a clipped piece of one (split_references)
+    std::vector<BBox> rb;
+    std::vector<uint32_t> rtri;
+    split_references(tris_in, pb, rb, rtri);
+    const size_t nr = rb.size();
+    if (nr >= (1u << 27)) {
+        err = "too many triangle references for the 27-bit leaf encoding";
+        return -1;
+    }
+    SahBuilder sb(rb);
     int32_t root = sb.alloc();
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     int par = 0;
     while ((1u << par) < hw && par < 6) ++par;
-    sb.build(root, 0, (uint32_t)n, par);
+    sb.build(root, 0, (uint32_t)nr, par);
     std::vector<BNode>& bn = sb.nodes;
 
-    // triangles in leaf order: leaves of the binary tree reference ranges of sb.idx
-    tris_out.resize(n);
-    for (size_t i = 0; i < n; ++i) tris_out[i] = tris_in[sb.idx[i]];
+    // triangle records in leaf order (a split triangle appears once per piece)
+    tris_out.resize(nr);
+    for (size_t i = 0; i < nr; ++i) tris_out[i] = tris_in[rtri[sb.idx[i]]];
 
     // collapse to 4-wide: children of a wide node = up to 4 binary nodes
     struct Pending {
