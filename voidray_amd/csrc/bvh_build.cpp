@@ -6,13 +6,16 @@
 //    exactly as merge_optionals (bvh.rs:164-181) does, and the exact
 //    reference engine that re-traces the rare rays whose closest hit cannot be
 //    confirmed by the fast engine.
-//  * build_qbvh: a binned-SAH binary BVH collapsed to 4-wide nodes with 8-bit
-//    outward-rounded child boxes (QNode, 64 B) and triangles in leaf order.
+//  * build_qbvh: a binned-SAH binary BVH with spatial splits collapsed to
+//    4-wide nodes with 8-bit outward-rounded child boxes (QNode, 64 B) and
+//    triangle records in leaf order.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
 #include <future>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 #include "bvh_build.h"
@@ -174,121 +177,6 @@ struct BNode {
     uint32_t start, count;
 };
 
-struct SahBuilder {
-    const std::vector<BBox>& pb;   // prim boxes
-    std::vector<float> cen;        // 3 per prim
-    std::vector<uint32_t> idx;
-    std::vector<BNode> nodes;
-    std::atomic<int32_t> next{0};
-    static constexpr int kBins = 16;
-
-    explicit SahBuilder(const std::vector<BBox>& b) : pb(b) {
-        size_t n = b.size();
-        cen.resize(3 * n);
-        idx.resize(n);
-        for (size_t i = 0; i < n; ++i) {
-            for (int a = 0; a < 3; ++a) cen[3 * i + a] = 0.5f * (b[i].lo[a] + b[i].hi[a]);
-            idx[i] = (uint32_t)i;
-        }
-        nodes.resize(std::max<size_t>(1, 2 * n));
-    }
-
-    int32_t alloc() { return next.fetch_add(1); }
-
-    void build(int32_t ni, uint32_t start, uint32_t count, int par) {
-        BNode& nd = nodes[ni];
-        BBox box, cb;
-        box.empty();
-        cb.empty();
-        for (uint32_t i = start; i < start + count; ++i) {
-            box.grow(pb[idx[i]]);
-            cb.grow(&cen[3 * idx[i]]);
-        }
-        nd.box = box;
-        nd.child[0] = nd.child[1] = -1;
-        nd.start = start;
-        nd.count = count;
-        if (count <= 1) return;
-        // binned SAH
-        float best_cost = kInf;
-        int best_axis = -1, best_split = -1;
-        for (int a = 0; a < 3; ++a) {
-            float ext = cb.hi[a] - cb.lo[a];
-            if (!(ext > 0.f)) continue;
-            BBox bb[kBins];
-            uint32_t bc[kBins] = {0};
-            for (auto& x : bb) x.empty();
-            float k = kBins * (1.f - 1e-6f) / ext;
-            for (uint32_t i = start; i < start + count; ++i) {
-                uint32_t p = idx[i];
-                int bin = (int)((cen[3 * p + a] - cb.lo[a]) * k);
-                bin = std::min(std::max(bin, 0), kBins - 1);
-                bb[bin].grow(pb[p]);
-                ++bc[bin];
-            }
-            float rarea[kBins];
-            uint32_t rcnt[kBins];
-            BBox acc;
-            acc.empty();
-            uint32_t c = 0;
-            for (int i = kBins - 1; i > 0; --i) {
-                acc.grow(bb[i]);
-                c += bc[i];
-                rarea[i] = acc.area();
-                rcnt[i] = c;
-            }
-            acc.empty();
-            c = 0;
-            for (int i = 0; i < kBins - 1; ++i) {
-                acc.grow(bb[i]);
-                c += bc[i];
-                if (c == 0 || rcnt[i + 1] == 0) continue;
-                float cost = acc.area() * (float)c + rarea[i + 1] * (float)rcnt[i + 1];
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_axis = a;
-                    best_split = i;
-                }
-            }
-        }
-        float leaf_cost = box.area() * (float)count;
-        float parent_area = box.area();
-        bool split_ok = best_axis >= 0;
-        if (split_ok && count <= kMaxLeaf) {
-            // SAH: traversal cost ~1 box test vs 1 triangle test per prim
-            float split_cost = parent_area * 1.0f + best_cost;
-            if (split_cost >= leaf_cost) return;
-        }
-        uint32_t mid;
-        if (split_ok) {
-            int a = best_axis;
-            float ext = cb.hi[a] - cb.lo[a];
-            float k = kBins * (1.f - 1e-6f) / ext;
-            auto it = std::partition(idx.begin() + start, idx.begin() + start + count, [&](uint32_t p) {
-                int bin = (int)((cen[3 * p + a] - cb.lo[a]) * k);
-                bin = std::min(std::max(bin, 0), kBins - 1);
-                return bin <= best_split;
-            });
-            mid = (uint32_t)(it - idx.begin());
-        } else {
-            if (count <= kMaxLeaf) return;
-            mid = start + count / 2;  // all centroids equal: split in the middle
-        }
-        if (mid == start || mid == start + count) mid = start + count / 2;
-        int32_t l = alloc(), r = alloc();
-        nd.child[0] = l;
-        nd.child[1] = r;
-        if (par > 0 && count > 32768) {
-            auto fut = std::async(std::launch::async, [&] { build(r, mid, start + count - mid, par - 1); });
-            build(l, start, mid - start, par - 1);
-            fut.get();
-        } else {
-            build(l, start, mid - start, 0);
-            build(r, mid, start + count - mid, 0);
-        }
-    }
-};
-
 // Outward padding of a triangle (or piece) box by 8 ulps of the coordinate
 // magnitude: absorbs the rounding of the slab test's subtraction.
 static void pad_box(BBox& b) {
@@ -300,27 +188,31 @@ static void pad_box(BBox& b) {
     }
 }
 
-// ---- early split clipping (Ernst & Greiner 2007) ---------------------------
-// A long, thin triangle lying diagonally has a box far larger than itself, and
-// every ray through that empty space visits its leaf.  Such a triangle is
-// referenced from several leaves instead: its box is cut at the midpoint of
-// the longest axis, the triangle is clipped to each half (in double), and each
-// piece gets the box of its clipped polygon, recursively while a cut removes
-// enough box area.  The pieces' boxes cover the whole triangle (each point of
-// it lies in one half, and each piece box is rounded outward and padded), so a
-// ray that hits the triangle reaches at least one leaf holding it; duplicate
-// records of a triangle give the same t and rank, so the result is unchanged.
+static float overlap_area(const BBox& a, const BBox& b) {
+    BBox c;
+    for (int k = 0; k < 3; ++k) {
+        c.lo[k] = std::max(a.lo[k], b.lo[k]);
+        c.hi[k] = std::min(a.hi[k], b.hi[k]);
+        if (!(c.lo[k] <= c.hi[k])) return 0.f;
+    }
+    return c.area();
+}
+
+// ---- triangle clipping (double) ---------------------------------------------
 struct DV3 {
     double v[3];
 };
+constexpr int kMaxPoly = 32;
 
-static int clip_half(const DV3* in, int n, int axis, double plane, bool keep_below, DV3* out) {
+// Sutherland-Hodgman against the closed half-space x[axis] <= plane (below) or
+// x[axis] >= plane; returns the vertex count of the clipped polygon.
+static int clip_half(const DV3* in, int n, int axis, double plane, bool below, DV3* out) {
     int m = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && m < kMaxPoly - 2; ++i) {
         const DV3& a = in[i];
         const DV3& b = in[(i + 1) % n];
-        double da = keep_below ? plane - a.v[axis] : a.v[axis] - plane;
-        double db = keep_below ? plane - b.v[axis] : b.v[axis] - plane;
+        double da = below ? plane - a.v[axis] : a.v[axis] - plane;
+        double db = below ? plane - b.v[axis] : b.v[axis] - plane;
         if (da >= 0) out[m++] = a;
         if ((da >= 0) != (db >= 0)) {
             double t = da / (da - db);
@@ -333,123 +225,301 @@ static int clip_half(const DV3* in, int n, int axis, double plane, bool keep_bel
     return m;
 }
 
-static double dbox_area(const double lo[3], const double hi[3]) {
-    double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-    return 2.0 * (dx * dy + dy * dz + dz * dx);
+// Float box (rounded outward, padded) of the part of triangle `t` inside `box`
+// (closed); false when that part is empty.  Never larger than `box`.
+static bool clipped_box(const TriRec& t, const BBox& box, BBox& out) {
+    DV3 a[kMaxPoly], b[kMaxPoly];
+    for (int k = 0; k < 3; ++k) {
+        a[0].v[k] = t.p0[k];
+        a[1].v[k] = t.p1[k];
+        a[2].v[k] = t.p2[k];
+    }
+    int n = 3;
+    for (int ax = 0; ax < 3 && n > 0; ++ax) {
+        n = clip_half(a, n, ax, (double)box.lo[ax], false, b);
+        if (n > 0) n = clip_half(b, n, ax, (double)box.hi[ax], true, a);
+    }
+    if (n == 0) return false;
+    for (int k = 0; k < 3; ++k) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int i = 0; i < n; ++i) {
+            lo = std::min(lo, a[i].v[k]);
+            hi = std::max(hi, a[i].v[k]);
+        }
+        float fl = (float)lo, fh = (float)hi;
+        if ((double)fl > lo) fl = std::nextafter(fl, -INFINITY);
+        if ((double)fh < hi) fh = std::nextafter(fh, INFINITY);
+        out.lo[k] = fl;
+        out.hi[k] = fh;
+    }
+    pad_box(out);
+    for (int k = 0; k < 3; ++k) {
+        out.lo[k] = std::max(out.lo[k], box.lo[k]);
+        out.hi[k] = std::min(out.hi[k], box.hi[k]);
+    }
+    return true;
 }
 
-static void poly_bounds(const DV3* p, int n, double lo[3], double hi[3]) {
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = INFINITY;
-        hi[a] = -INFINITY;
+// ---- binned SAH with spatial splits (SBVH: Stich, Friedrich & Dietrich 2009)
+// A node is split by the cheaper of an object split (binned on reference
+// centroids) and, where the object split's children overlap by more than
+// kAlpha of the root's area, a spatial split: a plane cuts the node and every
+// reference straddling it is clipped into a piece on each side.  A long thin
+// diagonal triangle therefore ends up in several leaves, each with the box of
+// its clipped piece, instead of one leaf whose box is mostly empty space.
+// Every point of a triangle lies in the piece on its side of each cut, and
+// each piece box is rounded outward and padded, so a ray that hits the
+// triangle reaches at least one leaf holding it; the copies of a triangle give
+// the same t and rank, so the closest hit does not change.
+struct SRef {
+    BBox box;
+    uint32_t tri;
+};
+
+struct SNode {
+    BBox box;
+    SNode* child[2];
+    uint32_t* tris;  // leaf: triangle indices (arena storage)
+    uint32_t count;
+};
+
+// Bump allocator owned by one build task.
+struct Arena {
+    static constexpr size_t kNodes = 1 << 14, kIds = 1 << 16;
+    std::vector<std::unique_ptr<SNode[]>> nodes;
+    std::vector<std::unique_ptr<uint32_t[]>> ids;
+    size_t nused = kNodes, iused = kIds;
+    SNode* node() {
+        if (nused == kNodes) {
+            nodes.emplace_back(new SNode[kNodes]);
+            nused = 0;
+        }
+        return &nodes.back()[nused++];
     }
-    for (int i = 0; i < n; ++i)
+    uint32_t* take_ids(uint32_t c) {
+        if (iused + c > kIds) {
+            ids.emplace_back(new uint32_t[kIds]);
+            iused = 0;
+        }
+        uint32_t* p = &ids.back()[iused];
+        iused += c;
+        return p;
+    }
+};
+
+struct SpatialBuilder {
+    const std::vector<TriRec>& tris;
+    float root_area = 0.f;
+    std::mutex mu;
+    std::vector<std::unique_ptr<Arena>> arenas;
+    static constexpr int kObjBins = 16;
+    static constexpr int kSpatialBins = 16;
+    static constexpr float kAlpha = 1e-4f;  // overlap (fraction of the root area) that enables spatial splits
+    static constexpr int kMaxDepth = 64;    // spatial splits stop below this depth
+
+    explicit SpatialBuilder(const std::vector<TriRec>& t) : tris(t) {}
+
+    Arena* new_arena() {
+        std::lock_guard<std::mutex> g(mu);
+        arenas.emplace_back(new Arena);
+        return arenas.back().get();
+    }
+
+    SNode* make_leaf(const std::vector<SRef>& refs, const BBox& box, Arena& ar) {
+        SNode* nd = ar.node();
+        nd->box = box;
+        nd->child[0] = nd->child[1] = nullptr;
+        nd->count = (uint32_t)refs.size();
+        nd->tris = ar.take_ids(nd->count);
+        for (size_t i = 0; i < refs.size(); ++i) nd->tris[i] = refs[i].tri;
+        return nd;
+    }
+
+    SNode* build(std::vector<SRef>& refs, Arena& ar, int depth, int par) {
+        const uint32_t n = (uint32_t)refs.size();
+        BBox box, cb;
+        box.empty();
+        cb.empty();
+        for (const SRef& r : refs) {
+            box.grow(r.box);
+            float c[3];
+            for (int a = 0; a < 3; ++a) c[a] = 0.5f * (r.box.lo[a] + r.box.hi[a]);
+            cb.grow(c);
+        }
+        if (depth == 0) root_area = box.area();
+        if (n <= 1) return make_leaf(refs, box, ar);
+
+        // object split: binned SAH over reference centroids
+        float obj_cost = kInf;
+        int obj_axis = -1, obj_split = -1;
+        BBox obj_l, obj_r;
         for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], p[i].v[a]);
-            hi[a] = std::max(hi[a], p[i].v[a]);
-        }
-}
-
-constexpr int kSplitDepth = 2;          // at most 4 pieces per triangle
-constexpr double kSplitKeep = 0.7;      // cut only if the halves keep < 70% of the box area
-constexpr double kSplitRatio = 14.0;    // split only triangles whose box area exceeds 14x their area
-
-static double tri_area(const DV3* p) {
-    double e1[3], e2[3];
-    for (int a = 0; a < 3; ++a) {
-        e1[a] = p[1].v[a] - p[0].v[a];
-        e2[a] = p[2].v[a] - p[0].v[a];
-    }
-    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
-    return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-}
-
-static void split_piece(const DV3* poly, int n, int depth, const BBox& tri_box, std::vector<BBox>& out) {
-    double lo[3], hi[3];
-    poly_bounds(poly, n, lo, hi);
-    // An axis-aligned right triangle has a box of 4x its area and a tilted
-    // grid triangle (a heightfield's) about 6-12x: those are left whole.
-    bool emit = depth >= kSplitDepth || (depth == 0 && !(dbox_area(lo, hi) > kSplitRatio * tri_area(poly)));
-    DV3 pl[9], pr[9];
-    int nl = 0, nr = 0;
-    if (!emit) {
-        int axis = 0;
-        for (int a = 1; a < 3; ++a)
-            if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
-        double mid = 0.5 * (lo[axis] + hi[axis]);
-        nl = clip_half(poly, n, axis, mid, true, pl);
-        nr = clip_half(poly, n, axis, mid, false, pr);
-        if (nl < 3 || nr < 3) {
-            emit = true;
-        } else {
-            double ll[3], lh[3], rl[3], rh[3];
-            poly_bounds(pl, nl, ll, lh);
-            poly_bounds(pr, nr, rl, rh);
-            if (dbox_area(ll, lh) + dbox_area(rl, rh) > kSplitKeep * dbox_area(lo, hi)) emit = true;
-        }
-    }
-    if (emit) {
-        BBox b;
-        for (int a = 0; a < 3; ++a) {
-            float fl = (float)lo[a], fh = (float)hi[a];
-            if ((double)fl > lo[a]) fl = std::nextafter(fl, -INFINITY);
-            if ((double)fh < hi[a]) fh = std::nextafter(fh, INFINITY);
-            b.lo[a] = fl;
-            b.hi[a] = fh;
-        }
-        pad_box(b);
-        for (int a = 0; a < 3; ++a) {  // never larger than the whole triangle's box
-            b.lo[a] = std::max(b.lo[a], tri_box.lo[a]);
-            b.hi[a] = std::min(b.hi[a], tri_box.hi[a]);
-        }
-        out.push_back(b);
-        return;
-    }
-    split_piece(pl, nl, depth + 1, tri_box, out);
-    split_piece(pr, nr, depth + 1, tri_box, out);
-}
-
-static void split_references(const std::vector<TriRec>& tris, const std::vector<BBox>& pb, std::vector<BBox>& rb,
-                             std::vector<uint32_t>& rtri) {
-    const size_t n = tris.size();
-    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    size_t chunk = (n + hw - 1) / hw;
-    std::vector<std::vector<BBox>> boxes(hw);
-    std::vector<std::vector<uint32_t>> owners(hw);
-    std::vector<std::future<void>> jobs;
-    for (unsigned w = 0; w < hw; ++w) {
-        jobs.push_back(std::async(std::launch::async, [&, w] {
-            size_t b0 = w * chunk, b1 = std::min(n, b0 + chunk);
-            std::vector<BBox> tmp;
-            for (size_t i = b0; i < b1; ++i) {
-                DV3 poly[3];
-                for (int a = 0; a < 3; ++a) {
-                    poly[0].v[a] = tris[i].p0[a];
-                    poly[1].v[a] = tris[i].p1[a];
-                    poly[2].v[a] = tris[i].p2[a];
+            float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.f)) continue;
+            BBox bb[kObjBins];
+            uint32_t bc[kObjBins] = {0};
+            for (auto& x : bb) x.empty();
+            float k = kObjBins * (1.f - 1e-6f) / ext;
+            for (const SRef& r : refs) {
+                int bin = (int)((0.5f * (r.box.lo[a] + r.box.hi[a]) - cb.lo[a]) * k);
+                bin = std::min(std::max(bin, 0), kObjBins - 1);
+                bb[bin].grow(r.box);
+                ++bc[bin];
+            }
+            BBox rbox[kObjBins];
+            uint32_t rcnt[kObjBins];
+            BBox acc;
+            acc.empty();
+            uint32_t c = 0;
+            for (int i = kObjBins - 1; i > 0; --i) {
+                acc.grow(bb[i]);
+                c += bc[i];
+                rbox[i] = acc;
+                rcnt[i] = c;
+            }
+            acc.empty();
+            c = 0;
+            for (int i = 0; i < kObjBins - 1; ++i) {
+                acc.grow(bb[i]);
+                c += bc[i];
+                if (c == 0 || rcnt[i + 1] == 0) continue;
+                float cost = acc.area() * (float)c + rbox[i + 1].area() * (float)rcnt[i + 1];
+                if (cost < obj_cost) {
+                    obj_cost = cost;
+                    obj_axis = a;
+                    obj_split = i;
+                    obj_l = acc;
+                    obj_r = rbox[i + 1];
                 }
-                tmp.clear();
-                split_piece(poly, 3, 0, pb[i], tmp);
-                if (tmp.size() <= 1) {
-                    boxes[w].push_back(pb[i]);
-                    owners[w].push_back((uint32_t)i);
-                } else {
-                    for (auto& b : tmp) {
-                        boxes[w].push_back(b);
-                        owners[w].push_back((uint32_t)i);
+            }
+        }
+
+        // spatial split where the object split leaves heavy overlap
+        float sp_cost = kInf;
+        int sp_axis = -1;
+        float sp_plane = 0.f;
+        bool try_spatial = depth < kMaxDepth &&
+                           (obj_axis < 0 || overlap_area(obj_l, obj_r) > kAlpha * root_area);
+        if (try_spatial) {
+            for (int a = 0; a < 3; ++a) {
+                float lo = box.lo[a], hi = box.hi[a];
+                float ext = hi - lo;
+                if (!(ext > 0.f)) continue;
+                float planes[kSpatialBins + 1];
+                for (int i = 0; i <= kSpatialBins; ++i) planes[i] = lo + ext * ((float)i / kSpatialBins);
+                planes[kSpatialBins] = hi;
+                BBox bb[kSpatialBins];
+                uint32_t enter[kSpatialBins] = {0}, leave[kSpatialBins] = {0};
+                for (auto& x : bb) x.empty();
+                float k = kSpatialBins * (1.f - 1e-6f) / ext;
+                for (const SRef& r : refs) {
+                    int b0 = std::min(std::max((int)((r.box.lo[a] - lo) * k), 0), kSpatialBins - 1);
+                    int b1 = std::min(std::max((int)((r.box.hi[a] - lo) * k), 0), kSpatialBins - 1);
+                    ++enter[b0];
+                    ++leave[b1];
+                    if (b0 == b1) {
+                        bb[b0].grow(r.box);
+                        continue;
+                    }
+                    for (int bi = b0; bi <= b1; ++bi) {
+                        BBox slab = r.box;
+                        slab.lo[a] = std::max(slab.lo[a], planes[bi]);
+                        slab.hi[a] = std::min(slab.hi[a], planes[bi + 1]);
+                        BBox piece;
+                        if (slab.lo[a] <= slab.hi[a] && clipped_box(tris[r.tri], slab, piece)) bb[bi].grow(piece);
+                    }
+                }
+                BBox rbox[kSpatialBins];
+                uint32_t rcnt[kSpatialBins];
+                BBox acc;
+                acc.empty();
+                uint32_t c = 0;
+                for (int i = kSpatialBins - 1; i > 0; --i) {
+                    acc.grow(bb[i]);
+                    c += leave[i];
+                    rbox[i] = acc;
+                    rcnt[i] = c;
+                }
+                acc.empty();
+                c = 0;
+                for (int i = 0; i < kSpatialBins - 1; ++i) {
+                    acc.grow(bb[i]);
+                    c += enter[i];
+                    if (c == 0 || rcnt[i + 1] == 0 || c >= n || rcnt[i + 1] >= n) continue;
+                    float cost = acc.area() * (float)c + rbox[i + 1].area() * (float)rcnt[i + 1];
+                    if (cost < sp_cost) {
+                        sp_cost = cost;
+                        sp_axis = a;
+                        sp_plane = planes[i + 1];
                     }
                 }
             }
-        }));
+        }
+
+        const bool use_spatial = sp_axis >= 0 && sp_cost < obj_cost;
+        const float best = use_spatial ? sp_cost : obj_cost;
+        if (n <= kMaxLeaf) {
+            // SAH: traversal cost ~1 box test vs 1 triangle test per reference
+            if (!(best < kInf) || box.area() * 1.0f + best >= box.area() * (float)n) return make_leaf(refs, box, ar);
+        }
+        std::vector<SRef> left, right;
+        if (use_spatial) {
+            const int a = sp_axis;
+            left.reserve(n);
+            right.reserve(n);
+            for (const SRef& r : refs) {
+                if (r.box.hi[a] <= sp_plane) {
+                    left.push_back(r);
+                } else if (r.box.lo[a] >= sp_plane) {
+                    right.push_back(r);
+                } else {
+                    BBox lb = r.box, rb = r.box, lp, rp;
+                    lb.hi[a] = sp_plane;
+                    rb.lo[a] = sp_plane;
+                    bool hl = clipped_box(tris[r.tri], lb, lp), hr = clipped_box(tris[r.tri], rb, rp);
+                    if (hl) left.push_back(SRef{lp, r.tri});
+                    if (hr) right.push_back(SRef{rp, r.tri});
+                    if (!hl && !hr) left.push_back(r);  // (numerically empty on both sides)
+                }
+            }
+            if (left.empty() || right.empty() || left.size() >= n + (size_t)n || right.size() >= n + (size_t)n) {
+                left.clear();
+                right.clear();
+            }
+        }
+        if (left.empty() && right.empty()) {
+            if (obj_axis >= 0) {
+                const int a = obj_axis;
+                float k = kObjBins * (1.f - 1e-6f) / (cb.hi[a] - cb.lo[a]);
+                for (const SRef& r : refs) {
+                    int bin = (int)((0.5f * (r.box.lo[a] + r.box.hi[a]) - cb.lo[a]) * k);
+                    bin = std::min(std::max(bin, 0), kObjBins - 1);
+                    (bin <= obj_split ? left : right).push_back(r);
+                }
+            }
+            if (left.empty() || right.empty()) {
+                if (n <= kMaxLeaf) return make_leaf(refs, box, ar);
+                left.assign(refs.begin(), refs.begin() + n / 2);  // all centroids equal
+                right.assign(refs.begin() + n / 2, refs.end());
+            }
+        }
+        std::vector<SRef>().swap(refs);
+        SNode* nd = ar.node();
+        nd->box = box;
+        nd->tris = nullptr;
+        nd->count = 0;
+        if (par > 0 && n > 32768) {
+            Arena* ra = new_arena();
+            auto fut = std::async(std::launch::async, [&, ra] { return build(right, *ra, depth + 1, par - 1); });
+            nd->child[0] = build(left, ar, depth + 1, par - 1);
+            nd->child[1] = fut.get();
+        } else {
+            nd->child[0] = build(left, ar, depth + 1, 0);
+            nd->child[1] = build(right, ar, depth + 1, 0);
+        }
+        return nd;
     }
-    for (auto& j : jobs) j.get();
-    rb.clear();
-    rtri.clear();
-    for (unsigned w = 0; w < hw; ++w) {
-        rb.insert(rb.end(), boxes[w].begin(), boxes[w].end());
-        rtri.insert(rtri.end(), owners[w].begin(), owners[w].end());
-    }
-}
+};
 
 // Per-axis power-of-two grid for a node: smallest e with 255 * 2^e >= extent.
 static int pick_exponent(float lo, float hi) {
@@ -473,8 +543,8 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
         err = "too many triangles for the 27-bit leaf encoding";
         return -1;
     }
-    // padded triangle boxes: outward by a few ulps of the coordinate magnitude
-    std::vector<BBox> pb(n);
+    // references: every triangle with its padded box
+    std::vector<SRef> refs(n);
     for (size_t i = 0; i < n; ++i) {
         BBox b;
         b.empty();
@@ -482,28 +552,52 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
         b.grow(tris_in[i].p1);
         b.grow(tris_in[i].p2);
         pad_box(b);
-        pb[i] = b;
+        refs[i] = SRef{b, (uint32_t)i};
     }
-    // references: a triangle, or a clipped piece of one (split_references)
-    std::vector<BBox> rb;
-    std::vector<uint32_t> rtri;
-    split_references(tris_in, pb, rb, rtri);
-    const size_t nr = rb.size();
-    if (nr >= (1u << 27)) {
-        err = "too many triangle references for the 27-bit leaf encoding";
-        return -1;
-    }
-    SahBuilder sb(rb);
-    int32_t root = sb.alloc();
+    SpatialBuilder sb(tris_in);
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     int par = 0;
     while ((1u << par) < hw && par < 6) ++par;
-    sb.build(root, 0, (uint32_t)nr, par);
-    std::vector<BNode>& bn = sb.nodes;
+    Arena* ra = sb.new_arena();
+    SNode* sroot = sb.build(refs, *ra, 0, par);
 
-    // triangle records in leaf order (a split triangle appears once per piece)
-    tris_out.resize(nr);
-    for (size_t i = 0; i < nr; ++i) tris_out[i] = tris_in[rtri[sb.idx[i]]];
+    // flatten in pre-order; triangle records in leaf order (a triangle split
+    // by spatial cuts appears once per leaf holding a piece of it)
+    std::vector<BNode> bn;
+    std::vector<uint32_t> order;
+    bn.reserve(2 * n);
+    order.reserve(n + n / 2);
+    {
+        std::vector<std::pair<SNode*, int32_t>> st;  // node, slot of the parent's child link (-1: root)
+        st.push_back({sroot, -1});
+        while (!st.empty()) {
+            auto [sn, link] = st.back();
+            st.pop_back();
+            int32_t me = (int32_t)bn.size();
+            if (link >= 0) bn[link >> 1].child[link & 1] = me;
+            BNode b;
+            b.box = sn->box;
+            b.child[0] = b.child[1] = -1;
+            b.start = b.count = 0;
+            if (!sn->child[0]) {
+                b.start = (uint32_t)order.size();
+                b.count = sn->count;
+                order.insert(order.end(), sn->tris, sn->tris + sn->count);
+            }
+            bn.push_back(b);
+            if (sn->child[0]) {
+                st.push_back({sn->child[1], 2 * me + 1});
+                st.push_back({sn->child[0], 2 * me});
+            }
+        }
+    }
+    if (order.size() >= (1u << 27)) {
+        err = "too many triangle references for the 27-bit leaf encoding";
+        return -1;
+    }
+    const int32_t root = 0;
+    tris_out.resize(order.size());
+    for (size_t i = 0; i < order.size(); ++i) tris_out[i] = tris_in[order[i]];
 
     // collapse to 4-wide: children of a wide node = up to 4 binary nodes
     struct Pending {
