@@ -9,9 +9,9 @@ constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave, 5 waves/SIMD
 constexpr int kLdsStackAny = 8;  // any-hit: 4 KB per wave, so registers (7 waves/SIMD) bound occupancy
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
-constexpr int kRefill = 16;      // idle lanes that trigger a wave-wide refill (closest hit)
-constexpr int kRefillAny = 32;   // the same for any-hit queries (shorter rays: refill less often)
-constexpr int kNodeSteps = 3;    // most node-phase iterations between two leaf phases
+constexpr int kRefill = 40;      // idle lanes that trigger a wave-wide refill (closest hit)
+constexpr int kRefillAny = 56;   // the same for any-hit queries (shorter rays: refill less often)
+constexpr int kNodeSteps = 4;    // most node-phase iterations between two leaf phases
 
 struct TraceArgs {
     SceneView sv;
@@ -48,6 +48,9 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
 // followed by a leaf phase (one leaf's triangles); whenever kRefill lanes of
 // the wave are idle the wave fetches that many new rays with one atomic
 // (dynamic ray fetch), so long incoherent rays do not leave the wave idle.
+// A refill costs an atomic round trip plus the new rays' loads, and with a
+// spatial-split BVH a ray needs only ~20 steps, so refills wait until most of
+// the wave is idle (40 of 64 lanes; 56 for the shorter any-hit rays).
 template <bool ANY>
 __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
