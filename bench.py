@@ -24,12 +24,14 @@ BYTES_PER_RAY_ANY = 25
 HBM_PEAK_GBS = 8000.0
 
 
-def _traffic_from_profiles():
+def _counters_from_profiles():
+    """rocprofv3 PMC figures for the traversal kernel (committed summary of the
+    FETCH_SIZE / WRITE_SIZE / TCC / SQ passes over this same workload)."""
     p = os.path.join(ROOT, "profiles", "trace_wide_microbench_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    return None
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        return json.load(f)
 
 
 def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
@@ -443,6 +445,7 @@ def main():
 
     mrays = n / (ms * 1e-3) / 1e6
     achieved = BYTES_PER_RAY_CLOSEST * n / (kernel_ms * 1e-3) / 1e9
+    pmc = _counters_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     serving = None if args.no_serving else _serving_leg(ctx, torch, stream)
@@ -461,12 +464,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "traversal microbench", "triangles": args.tris, "rays": n,
-                   "rays_mix": "1/2 coherent 2048^2 pinhole, 1/2 incoherent", "bvh": "SAH 4-wide, 8-bit quantised",
+                   "rays_mix": "1/2 coherent 2048^2 pinhole, 1/2 incoherent", "bvh": "SAH + spatial splits, 4-wide, 8-bit quantised",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _traffic_from_profiles(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": pmc.get("hbm_bytes_per_launch"),
                      "kernel": "k_trace_persist<false>", "kernel_ms": round(kernel_ms, 4),
-                     "bytes_per_ray": BYTES_PER_RAY_CLOSEST},
+                     "bytes_per_ray": BYTES_PER_RAY_CLOSEST,
+                     # rocprofv3 PMC passes over the same workload (profiles/)
+                     "traffic_bytes_per_ray": pmc.get("hbm_bytes_per_ray"),
+                     "l2_hit_rate": pmc.get("l2_hit_rate"),
+                     "occupancy_waves_per_simd": pmc.get("occupancy_waves_per_simd"),
+                     "valu_busy": pmc.get("valu_busy"),
+                     "valu_lane_utilization": pmc.get("valu_lane_utilization")},
         "cpu_baseline": cpu,
         "extra": {
             "coherent_mrays_s": round(half / (ms_coh * 1e-3) / 1e6, 2),
