@@ -71,7 +71,7 @@ def _terrain(ctx):
     return _TERRAIN["scene"], _TERRAIN["accel"]
 
 
-def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096):
+def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
     + MIS) at width x height, judged against the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
@@ -164,7 +164,7 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096):
                                                                               sample_offset=off, seed=7, flags=ext,
                                                                               rr_depth=4),
                                    train_paths=1 << 16, train_vertices=4, cache_depth=depth,
-                                   train_steps=8 if trained else 0, frame=k, stream=sp)
+                                   train_steps=train_steps if trained else 0, frame=k, stream=sp)
                 return n
 
             pts = curve(warm + frames, cached_pass, frames * spf)
@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--only-serving", action="store_true", help="run the batched-serving leg alone (prints it)")
     ap.add_argument("--no-cache", action="store_true", help="skip the learned-radiance-cache leg")
     ap.add_argument("--only-cache", action="store_true", help="run the learned-radiance-cache leg alone (prints it)")
+    ap.add_argument("--cache-train-steps", type=int, default=8,
+                    help="Adam steps per cached frame (disjoint slices of the frame's 2^16 samples)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 closest-hit queries of the workload, then a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
@@ -368,7 +370,8 @@ def main():
     if args.only_cache:
         st = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(st)
-        print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st)}), flush=True)
+        print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st, train_steps=args.cache_train_steps)}),
+              flush=True)
         return
     if args.only_serving:
         st = torch.cuda.Stream(device=dev)
@@ -449,7 +452,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     serving = None if args.no_serving else _serving_leg(ctx, torch, stream)
-    cache = None if args.no_cache else _cache_leg(ctx, torch, stream)
+    cache = None if args.no_cache else _cache_leg(ctx, torch, stream, train_steps=args.cache_train_steps)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),

@@ -9,8 +9,8 @@ constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave, 5 waves/SIMD
 constexpr int kLdsStackAny = 8;  // any-hit: 4 KB per wave, so registers (7 waves/SIMD) bound occupancy
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
-constexpr int kRefill = 40;      // idle lanes that trigger a wave-wide refill (closest hit)
-constexpr int kRefillAny = 56;   // the same for any-hit queries (shorter rays: refill less often)
+constexpr int kRefill = 24;      // idle lanes that trigger a wave-wide refill (closest hit)
+constexpr int kRefillAny = 40;   // the same for any-hit queries (shorter rays: refill less often)
 constexpr int kNodeSteps = 4;    // most node-phase iterations between two leaf phases
 
 struct TraceArgs {
@@ -27,7 +27,7 @@ struct TraceArgs {
     uint8_t* hit;
     uint32_t* fb_list;
     uint32_t* fb_count;
-    uint32_t* next;       // ray counter for dynamic fetch
+    uint32_t* next;       // partition counters of the dynamic ray fetch (kFetchStride apart)
     u32x2* spill;         // [kSpillCap][spill_lanes]
     uint32_t spill_lanes;
     const uint32_t* queue;  // optional: item k traces ray queue[k] (else ray k)
@@ -43,14 +43,40 @@ __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const 
     if (a.hit) a.hit[i] = h.prim != 0xFFFFFFFFu ? 1 : 0;
 }
 
+// Words of a scene's trace counter block.  The wide kernel appends to the
+// fallback list and fetches rays through the partition counters; the
+// cooperative exact kernel appends overflows, counts finished blocks, and its
+// last block publishes the launch's counts and clears the working words, so
+// every launch starts from zero without a reset launch (or a memset node in a
+// captured graph).
+enum : int {
+    kCtrFallbacks = 0,      // rays handed from the wide engine to the exact engine
+    kCtrOverflow = 2,       // rays whose frontier outgrew the cooperative LDS stack
+    kCtrBlocksDone = 3,     // finished blocks of the cooperative exact kernel
+    kCtrLastFallbacks = 8,  // published: fallbacks of the last launch
+    kCtrLastOverflow = 9,   // published: overflows of the last launch
+    kCtrFetch = 64,         // first ray-fetch partition counter
+    kCtrWords = 1024,
+};
+// Dynamic ray fetch draws from kFetchParts partitions of the ray range, each
+// with its own counter on its own 128-byte line: with ~20 traversal steps per
+// ray, thousands of waves refilling through one counter serialise on that
+// address.  A wave starts on partition blockIdx % kFetchParts and moves on
+// when it runs dry.
+constexpr int kFetchParts = 16;
+constexpr int kFetchStride = 32;  // words between two partition counters
+static_assert(kCtrFetch + kFetchParts * kFetchStride <= kCtrWords, "counter block too small");
+static_assert(kFetchParts <= 64, "one lane clears each partition counter");
+
 // Persistent wide-engine kernel: every lane runs its own ray as a resumable
 // state machine; each iteration is a node phase (up to kNodeSteps inner nodes)
 // followed by a leaf phase (one leaf's triangles); whenever kRefill lanes of
 // the wave are idle the wave fetches that many new rays with one atomic
 // (dynamic ray fetch), so long incoherent rays do not leave the wave idle.
 // A refill costs an atomic round trip plus the new rays' loads, and with a
-// spatial-split BVH a ray needs only ~20 steps, so refills wait until most of
-// the wave is idle (40 of 64 lanes; 56 for the shorter any-hit rays).
+// spatial-split BVH a ray needs only ~20 steps, so refills wait until a
+// good part of the wave is idle (24 of 64 lanes; 40 for the shorter any-hit
+// rays) and draw from partitioned counters (below).
 template <bool ANY>
 __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
@@ -63,18 +89,31 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
     WideState st;
     uint32_t ray = kNone;
     bool exhausted = false;
+    uint32_t part = blockIdx.x % kFetchParts;  // wave-uniform
+    int dry = 0;                               // partitions found empty so far
     for (;;) {
         bool idle = ray == kNone;
         uint64_t idle_mask = __ballot(idle);
         int nidle = __popcll(idle_mask);
         if (!exhausted && nidle >= (ANY ? kRefillAny : kRefill)) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(a.next, (uint32_t)nidle);
-            base = __shfl(base, 0);
-            if (base + (uint32_t)nidle >= n) exhausted = true;
-            if (idle) {
+            // take nidle rays from the current partition, moving on while it is empty
+            uint32_t base = 0, hi = 0;
+            for (;;) {
+                const uint32_t lo = (uint32_t)((uint64_t)n * part / kFetchParts);
+                hi = (uint32_t)((uint64_t)n * (part + 1) / kFetchParts);
+                uint32_t got = 0;
+                if (lane == 0) got = atomicAdd(a.next + part * kFetchStride, (uint32_t)nidle);
+                base = lo + __shfl(got, 0);
+                if (base < hi) break;
+                if (++dry == kFetchParts) {
+                    exhausted = true;
+                    break;
+                }
+                part = part + 1 == (uint32_t)kFetchParts ? 0u : part + 1;
+            }
+            if (idle && !exhausted) {
                 uint32_t k = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
-                if (k < n) {
+                if (k < hi) {
                     uint32_t r = a.queue ? a.queue[k] : k;
                     ray = r;
                     vf3 o = mk3(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2]);
@@ -129,20 +168,6 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
         }
     }
 }
-
-// Words of a scene's trace counter block.  The wide kernel appends to the
-// fallback list and fetches rays with kCtrNext; the cooperative exact kernel
-// appends overflows, counts finished blocks, and its last block publishes the
-// launch's counts and clears the working words, so every launch starts from
-// zero without a reset launch (or a memset node in a captured graph).
-enum : int {
-    kCtrFallbacks = 0,      // rays handed from the wide engine to the exact engine
-    kCtrNext = 1,           // dynamic ray fetch of the persistent kernel
-    kCtrOverflow = 2,       // rays whose frontier outgrew the cooperative LDS stack
-    kCtrBlocksDone = 3,     // finished blocks of the cooperative exact kernel
-    kCtrLastFallbacks = 8,  // published: fallbacks of the last launch
-    kCtrLastOverflow = 9,   // published: overflows of the last launch
-};
 
 __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -345,10 +370,10 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
         ctr[kCtrLastFallbacks] = fb;
         ctr[kCtrLastOverflow] = ns;
         ctr[kCtrFallbacks] = 0u;
-        ctr[kCtrNext] = 0u;
         ctr[kCtrOverflow] = 0u;
         ctr[kCtrBlocksDone] = 0u;
     }
+    if (lane < kFetchParts) ctr[kCtrFetch + lane * kFetchStride] = 0u;
 }
 
 static int persist_grid(vr_scene* s, bool any, uint32_t& blocks) {
@@ -378,11 +403,12 @@ int ensure_trace_scratch(vr_scene* s, uint32_t n) {
         if (hipMalloc((void**)&s->spill, bytes) != hipSuccess) return fail(VR_ERR_OOM, "spill stack allocation failed");
     }
     if (!s->counters) {
-        VR_HIP_CHECK(hipMalloc((void**)&s->counters, 64 * sizeof(uint32_t)));
+        VR_HIP_CHECK(hipMalloc((void**)&s->counters, kCtrWords * sizeof(uint32_t)));
         VR_HIP_CHECK(hipHostMalloc((void**)&s->host_counters, 64 * sizeof(uint32_t), hipHostMallocDefault));
         for (int k = 0; k < 64; ++k) s->host_counters[k] = 0;
         // starts at zero; from then on every launch leaves it at zero
-        VR_HIP_CHECK(hipMemcpy(s->counters, s->host_counters, 64 * sizeof(uint32_t), hipMemcpyHostToDevice));
+        std::vector<uint32_t> zeros(kCtrWords, 0u);
+        VR_HIP_CHECK(hipMemcpy(s->counters, zeros.data(), kCtrWords * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     // two lists of n entries: wide-engine fallbacks, then the serial-exact overflow list
     if (s->fallback_cap < 2 * n) {
@@ -400,7 +426,7 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
     int rc = ensure_trace_scratch(s, j.n_max);
     if (rc) return rc;
     TraceArgs a{s->view, j.o, j.d, j.n_max, j.tmin, j.any ? 1 : 0, j.prim, j.t, j.u, j.v, j.hit,
-                s->fallback_list, s->counters, s->counters + 1, s->spill, s->spill_lanes, j.queue, j.count, j.tmax};
+                s->fallback_list, s->counters, s->counters + kCtrFetch, s->spill, s->spill_lanes, j.queue, j.count, j.tmax};
     uint32_t* serial_list = s->fallback_list + s->fallback_cap / 2;
     uint64_t* accum = j.engine == VR_ENGINE_WIDE ? j.fallback_accum : nullptr;
     if (j.engine == VR_ENGINE_WIDE) {
