@@ -6,8 +6,8 @@
 namespace vr {
 
 constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
-constexpr int kLdsStack = 16;    // LDS stack entries per lane (8 B each): 8 KB per wave, 5 waves/SIMD
-constexpr int kLdsStackAny = 8;  // any-hit: 4 KB per wave, so registers (7 waves/SIMD) bound occupancy
+constexpr int kLdsStack = 12;    // LDS stack entries per lane (8 B each): 6 KB per wave, 6 waves/SIMD
+constexpr int kLdsStackAny = 8;  // (registers, not LDS, bound the resident waves)
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
 constexpr int kRefill = 24;      // idle lanes that trigger a wave-wide refill (closest hit)
 constexpr int kRefillAny = 40;   // the same for any-hit queries (shorter rays: refill less often)
@@ -78,7 +78,7 @@ static_assert(kFetchParts <= 64, "one lane clears each partition counter");
 // good part of the wave is idle (24 of 64 lanes; 40 for the shorter any-hit
 // rays) and draw from partitioned counters (below).
 template <bool ANY>
-__global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
+__device__ __forceinline__ void trace_persist(const TraceArgs& a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
     __shared__ u32x2 lds[LDS * kWave];
     const int lane = threadIdx.x;
@@ -167,6 +167,21 @@ __global__ __launch_bounds__(kWave) void k_trace_persist(TraceArgs a) {
             ray = kNone;
         }
     }
+}
+
+// Closest hit: registers held to what 8 waves per SIMD allow (the 6 KB LDS
+// stack then bounds residency at 6 waves; a 4 KB stack gave 8 waves and ~1%
+// more on short rays but spilled more on long ones).  Any hit keeps the
+// registers it wants (7 waves): held to 64 it spills.
+template <bool ANY>
+__global__ void k_trace_persist(TraceArgs a);
+template <>
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false>(TraceArgs a) {
+    trace_persist<false>(a);
+}
+template <>
+__global__ __launch_bounds__(kWave) void k_trace_persist<true>(TraceArgs a) {
+    trace_persist<true>(a);
 }
 
 __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {

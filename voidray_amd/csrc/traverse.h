@@ -143,15 +143,20 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     st.found = false;
 }
 
+// Slot of stack entry i in the LDS ring (i >= 0).
+template <int LDS_STACK>
+VR_HD int ring_slot(int i) {
+    return (LDS_STACK & (LDS_STACK - 1)) == 0 ? (i & (LDS_STACK - 1)) : (int)((unsigned)i % (unsigned)LDS_STACK);
+}
+
 template <int LDS_STACK>
 VR_HD bool wide_push(WideState& st, const WideStack& ws, u32x2 e) {
-    static_assert((LDS_STACK & (LDS_STACK - 1)) == 0, "the LDS ring is indexed with a mask");
     if (st.sp - st.lo == LDS_STACK) {
         if (st.lo >= ws.spill_cap) return false;
-        ws.spill[st.lo * ws.spill_stride] = ws.lds[(st.lo & (LDS_STACK - 1)) * ws.stride];
+        ws.spill[st.lo * ws.spill_stride] = ws.lds[ring_slot<LDS_STACK>(st.lo) * ws.stride];
         ++st.lo;
     }
-    ws.lds[(st.sp & (LDS_STACK - 1)) * ws.stride] = e;
+    ws.lds[ring_slot<LDS_STACK>(st.sp) * ws.stride] = e;
     ++st.sp;
     return true;
 }
@@ -163,7 +168,7 @@ VR_HD u32x2 wide_pop(WideState& st, const WideStack& ws) {
         st.lo = st.sp;
         return ws.spill[st.sp * ws.spill_stride];
     }
-    return ws.lds[(st.sp & (LDS_STACK - 1)) * ws.stride];
+    return ws.lds[ring_slot<LDS_STACK>(st.sp) * ws.stride];
 }
 
 // Pops the nearest pending entry that survives culling into st.cur; false
