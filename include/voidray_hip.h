@@ -135,6 +135,9 @@ typedef struct vr_scene_stats {
     double build_sah_seconds;   /* SAH build + collapse + quantisation   */
     double upload_seconds;
     uint64_t device_bytes;
+    uint32_t num_tri_records;   /* triangle records in the wide BVH's leaves: >= num_triangles,
+                                   a triangle cut by spatial splits has one per leaf holding a piece */
+    uint32_t reserved;
 } vr_scene_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -186,6 +189,17 @@ int vr_trace_device_async(vr_scene* scene, const vr_trace_desc* desc,
                           const float* d_origins, const float* d_directions,
                           uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
                           uint8_t* d_hit, void* stream);
+/* The same queries answered on the host, without a GPU: the scene is built
+ * into host memory exactly as vr_scene_upload builds it, and the rays run
+ * through the same traversal routines the kernels use (wide engine with its
+ * exact-engine fallback, or the exact engine), single-threaded.  For checking
+ * the BVH build and the traversal where no GPU is present.  stats and
+ * fallbacks (rays the wide engine handed to the exact engine) may be NULL.
+ * No counterpart in the reference. */
+int vr_trace_host(const vr_scene_desc* scene, const vr_trace_desc* desc,
+                  const float* origins, const float* directions,
+                  uint32_t* prim_id, float* t, float* u, float* v, uint8_t* hit,
+                  vr_scene_stats* stats, uint32_t* fallbacks);
 /* Rays of the last trace that needed the exact fallback engine (call after
  * the stream of that trace has been synchronised). */
 int vr_trace_last_fallbacks(vr_scene* scene, uint32_t* count);

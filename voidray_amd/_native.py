@@ -75,7 +75,8 @@ class vr_scene_stats(C.Structure):
     _fields_ = [("num_triangles", C.c_uint32), ("num_surfaces", C.c_uint32), ("num_qnodes", C.c_uint32),
                 ("num_ref_nodes", C.c_uint32), ("qbvh_depth", C.c_uint32), ("max_leaf_size", C.c_uint32),
                 ("build_ref_seconds", C.c_double), ("build_sah_seconds", C.c_double),
-                ("upload_seconds", C.c_double), ("device_bytes", C.c_uint64)]
+                ("upload_seconds", C.c_double), ("device_bytes", C.c_uint64),
+                ("num_tri_records", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class vr_trace_desc(C.Structure):
@@ -148,6 +149,8 @@ SIGNATURES = {
     "vr_scene_get_stats": (C.c_int, [_P, C.POINTER(vr_scene_stats)]),
     "vr_trace": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P]),
     "vr_trace_device_async": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_trace_host": (C.c_int, [C.POINTER(vr_scene_desc), C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P,
+                                C.POINTER(vr_scene_stats), C.POINTER(C.c_uint32)]),
     "vr_trace_last_fallbacks": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "vr_trace_last_timing": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "vr_render": (C.c_int, [_P, C.POINTER(vr_render_desc), _P, C.POINTER(vr_render_stats)]),

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "internal.h"
+#include "traverse.h"
 #include "render.h"
 
 namespace vr {
@@ -153,6 +154,53 @@ int vr_trace(vr_scene* s, const vr_trace_desc* desc, const float* o, const float
         if (v) VR_HIP_CHECK(hipMemcpyAsync(v, base + off_v, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
         if (hit) VR_HIP_CHECK(hipMemcpyAsync(hit, base + off_hit, (size_t)n, hipMemcpyDeviceToHost, st));
         VR_HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
+int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const float* o, const float* d,
+                  uint32_t* prim, float* t, float* u, float* v, uint8_t* hit, vr_scene_stats* stats,
+                  uint32_t* fallbacks) {
+    return guarded([&] {
+        if (!sd || !desc) return fail(VR_ERR_INVALID, "null argument");
+        if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
+        if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE)
+            return fail(VR_ERR_INVALID, "bad engine");
+        const uint32_t n = desc->num_rays;
+        if (n && (!o || !d)) return fail(VR_ERR_INVALID, "null ray buffers");
+        vr::HostScene H;
+        int rc = vr::build_host_scene(sd, H);
+        if (rc) return rc;
+        if (stats) {
+            *stats = H.stats;
+            stats->num_tri_records = (uint32_t)H.tris.size();
+        }
+        const vr::SceneView& sv = H.view;
+        const bool any = desc->mode == VR_TRACE_ANY;
+        constexpr int kStack = 16, kSpill = 240;
+        u32x2 lds[kStack], spill[kSpill];
+        const vr::WideStack ws{lds, 1, spill, 1, kSpill};
+        uint32_t fb = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const vf3 oi = mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), di = mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+            vr::HitOut h;
+            bool exact = desc->engine == VR_ENGINE_REFERENCE;
+            if (!exact) {
+                int r = any ? vr::trace_wide<kStack, true>(sv, oi, di, desc->t_min, kInf, ws, h)
+                            : vr::trace_wide<kStack, false>(sv, oi, di, desc->t_min, kInf, ws, h);
+                if (r == vr::kStepFallback) {
+                    exact = true;
+                    ++fb;
+                }
+            }
+            if (exact) vr::trace_exact(sv, oi, di, desc->t_min, h);  // any hit: t_max = inf, so any hit counts
+            if (prim) prim[i] = h.prim;
+            if (t) t[i] = h.t;
+            if (u) u[i] = h.u;
+            if (v) v[i] = h.v;
+            if (hit) hit[i] = h.prim != 0xFFFFFFFFu ? 1 : 0;
+        }
+        if (fallbacks) *fallbacks = fb;
         return 0;
     });
 }

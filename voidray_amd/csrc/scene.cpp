@@ -3,9 +3,11 @@
 // the per-mesh BVH build of Mesh::from_buffers (core/mesh.rs:76-116).
 //
 // HBM layout (all arrays 256-byte aligned, one allocation each):
-//   qnodes   QNode[num_qnodes]      64 B  SAH 4-wide, 8-bit outward-quantised
-//   tris     TriRec[T]              48 B  world vertices in leaf order + prim + rank
+//   qnodes   QNode[num_qnodes]      64 B  SAH 4-wide with spatial splits, 8-bit outward-quantised
+//   tris     TriRec[R]              48 B  world vertices in leaf order + prim + rank (R >= T:
+//                                         one record per leaf holding a piece of a split triangle)
 //   ref/top  RefNode[]              32 B  reference median-split trees (exact engine)
+//   ref_parent, prim_anc            4 B   reference-tree parents, per prim its leaf's parent
 //   per mesh triangle: idx (12 B), face normal (12 B), in-order rank (4 B)
 //   per vertex: position, normal (12 B each), uv (8 B)
 //   materials, textures (RGBA8 or RGB f32 texels), HDRI (RGB f32) + IS tables
@@ -323,6 +325,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     st.num_ref_nodes = (uint32_t)ref_nodes.size();
     st.qbvh_depth = depth;
     st.max_leaf_size = kMaxLeaf;
+    st.num_tri_records = (uint32_t)tris.size();
     st.build_ref_seconds = std::chrono::duration<double>(t_ref1 - t_ref0).count();
     st.build_sah_seconds = std::chrono::duration<double>(t_sah1 - t_ref1).count();
     return 0;

@@ -275,3 +275,29 @@ def tonemap(ctx, src, post, dst=None):
     dst = np.zeros_like(src) if dst is None else dst
     N.check(N.lib().vr_tonemap(ctx.handle, C.byref(post), src.ctypes.data, dst.ctypes.data))
     return dst
+
+
+def trace_host(scene, origins, directions, t_min=1e-5, any_hit=False, engine=N.VR_ENGINE_WIDE, build_flags=0):
+    """SceneAcceleration::hit answered on the host (no GPU): the scene is built
+    as for the device and the rays run through the kernels' traversal routines
+    on the CPU.  Returns prim/t/u/v (closest) or hit (any), plus the build
+    statistics and the number of rays the wide engine handed to the exact one."""
+    o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+    n = len(o)
+    desc, keep = scene.to_desc(build_flags)
+    tdesc = N.vr_trace_desc(n, t_min, N.VR_TRACE_ANY if any_hit else N.VR_TRACE_CLOSEST, engine)
+    prim = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    hit = np.zeros(n, np.uint8)
+    stats = N.vr_scene_stats()
+    fb = C.c_uint32(0)
+    N.check(N.lib().vr_trace_host(C.byref(desc), C.byref(tdesc), o.ctypes.data, d.ctypes.data, prim.ctypes.data,
+                                  t.ctypes.data, u.ctypes.data, v.ctypes.data, hit.ctypes.data, C.byref(stats),
+                                  C.byref(fb)))
+    out = {"hit": hit} if any_hit else {"prim": prim, "t": t, "u": u, "v": v}
+    out["stats"] = {k: getattr(stats, k) for k, _ in N.vr_scene_stats._fields_}
+    out["fallbacks"] = fb.value
+    return out
