@@ -316,6 +316,7 @@ struct SpatialBuilder {
     static constexpr int kSpatialBins = 16;
     static constexpr float kAlpha = 1e-4f;  // overlap (fraction of the root area) that enables spatial splits
     static constexpr int kMaxDepth = 64;    // spatial splits stop below this depth
+    static constexpr uint64_t kMaxDup = 8;  // references per triangle at most, over the whole tree
 
     explicit SpatialBuilder(const std::vector<TriRec>& t) : tris(t) {}
 
@@ -335,7 +336,12 @@ struct SpatialBuilder {
         return nd;
     }
 
-    SNode* build(std::vector<SRef>& refs, Arena& ar, int depth, int par) {
+    // `budget`: most references this subtree may hold (>= refs.size()).  A
+    // node's budget is shared by its children in proportion to their
+    // references, so the tree never holds more than kMaxDup references per
+    // triangle whatever the geometry, and the split decisions stay independent
+    // of thread timing.
+    SNode* build(std::vector<SRef>& refs, Arena& ar, int depth, int par, uint64_t budget) {
         const uint32_t n = (uint32_t)refs.size();
         BBox box, cb;
         box.empty();
@@ -398,7 +404,7 @@ struct SpatialBuilder {
         float sp_cost = kInf;
         int sp_axis = -1;
         float sp_plane = 0.f;
-        bool try_spatial = depth < kMaxDepth &&
+        bool try_spatial = depth < kMaxDepth && budget > n &&
                            (obj_axis < 0 || overlap_area(obj_l, obj_r) > kAlpha * root_area);
         if (try_spatial) {
             for (int a = 0; a < 3; ++a) {
@@ -482,7 +488,7 @@ struct SpatialBuilder {
                     if (!hl && !hr) left.push_back(r);  // (numerically empty on both sides)
                 }
             }
-            if (left.empty() || right.empty() || left.size() >= n + (size_t)n || right.size() >= n + (size_t)n) {
+            if (left.empty() || right.empty() || left.size() + right.size() > budget) {
                 left.clear();
                 right.clear();
             }
@@ -504,18 +510,20 @@ struct SpatialBuilder {
             }
         }
         std::vector<SRef>().swap(refs);
+        const uint64_t nl = left.size(), nr = right.size();
+        const uint64_t bl = budget * nl / (nl + nr), br = budget - bl;  // bl >= nl, br >= nr
         SNode* nd = ar.node();
         nd->box = box;
         nd->tris = nullptr;
         nd->count = 0;
         if (par > 0 && n > 32768) {
             Arena* ra = new_arena();
-            auto fut = std::async(std::launch::async, [&, ra] { return build(right, *ra, depth + 1, par - 1); });
-            nd->child[0] = build(left, ar, depth + 1, par - 1);
+            auto fut = std::async(std::launch::async, [&, ra] { return build(right, *ra, depth + 1, par - 1, br); });
+            nd->child[0] = build(left, ar, depth + 1, par - 1, bl);
             nd->child[1] = fut.get();
         } else {
-            nd->child[0] = build(left, ar, depth + 1, 0);
-            nd->child[1] = build(right, ar, depth + 1, 0);
+            nd->child[0] = build(left, ar, depth + 1, 0, bl);
+            nd->child[1] = build(right, ar, depth + 1, 0, br);
         }
         return nd;
     }
@@ -559,7 +567,7 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
     int par = 0;
     while ((1u << par) < hw && par < 6) ++par;
     Arena* ra = sb.new_arena();
-    SNode* sroot = sb.build(refs, *ra, 0, par);
+    SNode* sroot = sb.build(refs, *ra, 0, par, SpatialBuilder::kMaxDup * (uint64_t)n);
 
     // flatten in pre-order; triangle records in leaf order (a triangle split
     // by spatial cuts appears once per leaf holding a piece of it)
