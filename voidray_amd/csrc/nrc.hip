@@ -528,6 +528,34 @@ __global__ void k_adam(float* p, float* g, float* m, float* v, uint64_t n, float
     }
 }
 
+// The same lazy Adam over the hash grid, four entries per thread (n % 4 == 0,
+// 16-byte aligned arrays): most of the 2^24 grid parameters are untouched in
+// a step, so the pass is a read of the gradients; the arithmetic per touched
+// entry is k_adam's.
+__global__ void k_adam_grid4(float* p, float* g, float* m, float* v, uint64_t n4, float lr_t, float b1, float b2,
+                             float eps, _Float16* out_h) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x) {
+        const f32x4 g4 = ((const f32x4*)g)[i];
+        if (g4.x == 0.f && g4.y == 0.f && g4.z == 0.f && g4.w == 0.f) continue;
+        f32x4 p4 = ((f32x4*)p)[i], m4 = ((f32x4*)m)[i], v4 = ((f32x4*)v)[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float gi = g4[k];
+            if (gi == 0.f) continue;
+            const float mi = b1 * m4[k] + (1.f - b1) * gi;
+            const float vi = b2 * v4[k] + (1.f - b2) * (gi * gi);
+            p4[k] = p4[k] - lr_t * mi / (sqrtf(vi) + eps);
+            m4[k] = mi;
+            v4[k] = vi;
+            out_h[4 * i + k] = (_Float16)p4[k];
+        }
+        ((f32x4*)p)[i] = p4;
+        ((f32x4*)m)[i] = m4;
+        ((f32x4*)v)[i] = v4;
+        ((f32x4*)g)[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
 __global__ void k_round_copy(const float* p, uint64_t n, __bf16* out_bf, _Float16* out_h) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         if (out_bf) out_bf[i] = (__bf16)p[i];
@@ -640,7 +668,8 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
             a.hT = c->hT;
             a.dT = c->dT;
             a.cap = c->train_cap;
-            hipLaunchKernelGGL(k_nrc_train, dim3(mlp_blocks(n)), dim3(kFwdBlock), 0, st, a);
+            // a mini-batch holds at most ceil(n / parts) samples: no blocks beyond those
+            hipLaunchKernelGGL(k_nrc_train, dim3(mlp_blocks((n + parts - 1) / parts)), dim3(kFwdBlock), 0, st, a);
             hipLaunchKernelGGL(k_nrc_wgrad, dim3(kWgradBlocks, 5), dim3(256), 0, st, (const __bf16*)c->xT,
                                (const __bf16*)c->hT, (const __bf16*)c->dT, c->train_cap, d_n, n, part, parts,
                                c->part);
@@ -659,9 +688,10 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
     hipLaunchKernelGGL(k_adam, dim3(grid_launch(kNrcParams)), dim3(256), 0, st, c->w_master, c->w_grad, c->w_m,
                        c->w_v, (uint64_t)kNrcParams, lrw, (float)b1, (float)b2, c->desc.eps, c->w_bf,
                        (_Float16*)nullptr, 0);
-    hipLaunchKernelGGL(k_adam, dim3(grid_launch(c->info.grid_params)), dim3(256), 0, st, c->g_master, c->g_grad,
-                       c->g_m, c->g_v, c->info.grid_params, lrg, (float)b1, (float)b2, c->desc.eps, (__bf16*)nullptr,
-                       c->g_h, 1);
+    static_assert(kNrcLevels * 2 % 4 == 0, "grid parameters come in multiples of 4");
+    hipLaunchKernelGGL(k_adam_grid4, dim3(grid_launch(c->info.grid_params / 4)), dim3(256), 0, st, c->g_master,
+                       c->g_grad, c->g_m, c->g_v, c->info.grid_params / 4, lrg, (float)b1, (float)b2, c->desc.eps,
+                       c->g_h);
     VR_HIP_CHECK(hipGetLastError());
     return 0;
 }
