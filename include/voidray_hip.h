@@ -101,6 +101,23 @@ typedef struct vr_object {
     uint32_t material;
 } vr_object;
 
+/* Analytic surface (voidray_common/src/surfaces.rs:31-114): a sphere
+ * (center, radius) or a ground plane y = height.  A ray query that returns one
+ * reports its surface's single prim id and, in u and v, the hit record's uv:
+ * (0, 0) on a sphere, (x, z) of the hit point on a ground plane. */
+#define VR_ANALYTIC_SPHERE 0
+#define VR_ANALYTIC_GROUND_PLANE 1
+typedef struct vr_analytic {
+    int32_t kind;      /* VR_ANALYTIC_* */
+    float center[3];
+    float radius;
+    float height;
+} vr_analytic;
+
+/* surface_mesh[i] with this bit set: surface i is analytic[surface_mesh[i] & ~bit]
+ * (Scene::add_analytic_surface, scene.rs:118-124) */
+#define VR_SURFACE_ANALYTIC 0x80000000u
+
 /* Whole scene (core/scene.rs:36-44).  Surface i is mesh surface_mesh[i]; as in
  * SceneAcceleration::hit (scene.rs:182-185) the material of a hit on surface
  * i is objects[i].material, so objects must have at least num_surfaces
@@ -119,6 +136,8 @@ typedef struct vr_scene_desc {
     vr_environment environment;
     vr_camera camera;
     uint32_t build_flags; /* VR_BUILD_* */
+    uint32_t num_analytic;
+    const vr_analytic* analytic;
 } vr_scene_desc;
 
 #define VR_BUILD_DEFAULT 0
@@ -175,8 +194,9 @@ typedef struct vr_trace_desc {
 } vr_trace_desc;
 
 /* Closest-hit: prim_id = global triangle index (surfaces in order, triangles
- * in face order) or 0xFFFFFFFF on a miss; t, u, v = Möller–Trumbore values of
- * Triangle::hit (mesh.rs:144-188).  Any-hit: hit[i] = 1 iff the reference
+ * in face order, one prim per analytic surface) or 0xFFFFFFFF on a miss; t, u,
+ * v = Möller–Trumbore values of Triangle::hit (mesh.rs:144-188), or t and the
+ * hit record's uv of an analytic surface (see vr_analytic).  Any-hit: hit[i] = 1 iff the reference
  * query returns Some, prim_id/t/u/v may be NULL.
  * Host pointers: origins/directions are 3*n floats (xyz per ray, directions
  * already normalised as Ray::new does, ray.rs:12-17). */

@@ -166,7 +166,8 @@ template <class R> struct Material {
 
 template <class R> struct Scene {
     std::vector<Mesh<R>> meshes;
-    std::vector<uint32_t> surface_mesh;
+    std::vector<uint32_t> surface_mesh;  // VR_SURFACE_ANALYTIC | index for analytic surfaces
+    std::vector<vr_analytic> analytic;
     std::vector<vr_object> objects;
     std::vector<Material<R>> materials;
     std::vector<Texture<R>> textures;
@@ -182,6 +183,12 @@ template <class R> struct Scene {
     std::vector<Node<R>> top;
     int32_t top_root = -1;
     uint32_t num_prims = 0;
+    bool is_analytic(uint32_t surf) const { return (surface_mesh[surf] & VR_SURFACE_ANALYTIC) != 0; }
+    const vr_analytic& analytic_of(uint32_t surf) const { return analytic[surface_mesh[surf] & ~VR_SURFACE_ANALYTIC]; }
+    // prims of a surface: its triangles, or 1 for an analytic surface
+    uint32_t surface_prims(uint32_t surf) const {
+        return is_analytic(surf) ? 1u : (uint32_t)(meshes[surface_mesh[surf]].idx.size() / 3);
+    }
 };
 
 // Triangle bounds (mesh.rs:193-205): eps-expanded to 0.001.
@@ -245,6 +252,7 @@ template <class R> Scene<R>* make_scene(const vr_scene_desc* d) {
     s->meshes.resize(d->num_meshes);
     for (uint32_t i = 0; i < d->num_meshes; ++i) make_mesh(s->meshes[i], d->meshes[i]);
     s->surface_mesh.assign(d->surface_mesh, d->surface_mesh + d->num_surfaces);
+    if (d->num_analytic) s->analytic.assign(d->analytic, d->analytic + d->num_analytic);
     s->objects.assign(d->objects, d->objects + d->num_objects);
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const vr_material& m = d->materials[i];
@@ -275,12 +283,26 @@ template <class R> Scene<R>* make_scene(const vr_scene_desc* d) {
     std::vector<uint32_t> surf_base(d->num_surfaces);
     for (uint32_t i = 0; i < d->num_surfaces; ++i) {
         surf_base[i] = base;
-        base += (uint32_t)(s->meshes[s->surface_mesh[i]].idx.size() / 3);
+        base += s->surface_prims(i);
     }
     s->num_prims = base;
     // top-level BVH over surfaces, unexpanded mesh bounds (scene.rs:72-80, 176)
+    // and the analytic surfaces' own bounds (surfaces.rs:36-43, 106-113)
     std::vector<AABB<R>> sb(d->num_surfaces);
-    for (uint32_t i = 0; i < d->num_surfaces; ++i) sb[i] = s->meshes[s->surface_mesh[i]].bounds;
+    for (uint32_t i = 0; i < d->num_surfaces; ++i) {
+        if (!s->is_analytic(i)) {
+            sb[i] = s->meshes[s->surface_mesh[i]].bounds;
+            continue;
+        }
+        const vr_analytic& a = s->analytic_of(i);
+        if (a.kind == VR_ANALYTIC_SPHERE) {
+            V3<R> c = load3<R>(a.center, 0), rr = v3<R>(R(a.radius), R(a.radius), R(a.radius));
+            sb[i] = AABB<R>{c - rr, c + rr};
+        } else {
+            sb[i] = AABB<R>{v3<R>(-inf_v<R>(), R(a.height) - R(0.0001f), -inf_v<R>()),
+                            v3<R>(inf_v<R>(), R(a.height) + R(0.0001f), inf_v<R>())};
+        }
+    }
     std::vector<uint32_t> objs(d->num_surfaces);
     for (uint32_t i = 0; i < d->num_surfaces; ++i) objs[i] = i;
     s->top_root = build_ref(objs, sb, s->top);
@@ -380,6 +402,48 @@ template <class R> bool mesh_hit(const Mesh<R>& m, const Ray<R>& ray, R tmin, Hi
     return have;
 }
 
+// AnalyticSurface::hit with t_max = inf: Sphere::hit (surfaces.rs:46-78) and
+// GroundPlane::hit (surfaces.rs:88-104); u, v carry the hit record's uv.  A
+// ray lying exactly in a ground plane (t = 0/0) is taken as a miss.
+template <class R> bool analytic_hit(const vr_analytic& a, const Ray<R>& ray, R tmin, Hit<R>& out) {
+    R t;
+    V3<R> n;
+    V2<R> uv{R(0), R(0)};
+    if (a.kind == VR_ANALYTIC_SPHERE) {
+        V3<R> c = load3<R>(a.center, 0);
+        V3<R> oc = ray.o - c;
+        R aa = dot(ray.d, ray.d);
+        R half_b = dot(oc, ray.d);
+        R cc = dot(oc, oc) - R(a.radius) * R(a.radius);
+        R disc = half_b * half_b - aa * cc;
+        if (disc < R(0)) return false;
+        R sqrtd = std::sqrt(disc);
+        R root = (-half_b - sqrtd) / aa;
+        if (root < tmin || inf_v<R>() < root) {
+            root = (-half_b + sqrtd) / aa;
+            if (root < tmin || inf_v<R>() < root) return false;
+        }
+        t = root;
+        n = (ray.at(t) - c) / R(a.radius);
+    } else {
+        t = (R(a.height) - ray.o.y) / ray.d.y;
+        if (t <= tmin || t >= inf_v<R>() || t != t) return false;
+        V3<R> p = ray.at(t);
+        uv = V2<R>{p.x, p.z};
+        n = v3<R>(R(0), R(1), R(0));
+    }
+    // HitRecord::new (ray.rs:34-49)
+    out.point = ray.at(t);
+    out.front_face = dot(ray.d, n) < R(0);
+    out.normal = out.front_face ? n : -n;
+    out.t = t;
+    out.uv = uv;
+    out.u = uv.x;
+    out.v = uv.y;
+    out.tri = 0;
+    return true;
+}
+
 // SceneAcceleration::hit (scene.rs:182-185) via the top-level tree.
 template <class R> bool scene_hit(const Scene<R>& s, const Ray<R>& ray, R tmin, Hit<R>& best, bool prune, Counters& ctr) {
     if (s.top_root < 0) return false;
@@ -392,7 +456,8 @@ template <class R> bool scene_hit(const Scene<R>& s, const Ray<R>& ray, R tmin, 
         if (nd.obj >= 0) {
             uint32_t surf = (uint32_t)nd.obj;
             Hit<R> h;
-            if (mesh_hit(s.meshes[s.surface_mesh[surf]], ray, tmin, h, prune, ctr)) {
+            if (s.is_analytic(surf) ? analytic_hit(s.analytic_of(surf), ray, tmin, h)
+                                    : mesh_hit(s.meshes[s.surface_mesh[surf]], ray, tmin, h, prune, ctr)) {
                 if (!have || h.t <= best.t) {
                     h.surface = surf;
                     best = h;
@@ -882,7 +947,7 @@ template <class R> void trace_all(const Scene<R>* s, uint32_t n, const float* o,
     uint32_t base = 0;
     for (size_t i = 0; i < surf_base.size(); ++i) {
         surf_base[i] = base;
-        base += (uint32_t)(s->meshes[s->surface_mesh[i]].idx.size() / 3);
+        base += s->surface_prims((uint32_t)i);
     }
     std::atomic<uint32_t> next{0};
     std::vector<Counters> per(std::max(1, nthreads));
@@ -1019,6 +1084,17 @@ void orc_trace_bruteforce_f32(void* sp, uint32_t n, const float* o, const float*
             Hit<float> best;
             uint32_t best_p = 0, base = 0;
             for (size_t sf = 0; sf < s->surface_mesh.size(); ++sf) {
+                if (s->is_analytic((uint32_t)sf)) {
+                    Hit<float> h;
+                    if (analytic_hit(s->analytic_of((uint32_t)sf), r, tmin, h) &&
+                        (!have || h.t < best.t || (h.t == best.t && rank[base] > rank[best_p]))) {
+                        best = h;
+                        best_p = base;
+                        have = true;
+                    }
+                    base += 1;
+                    continue;
+                }
                 const Mesh<float>& m = s->meshes[s->surface_mesh[sf]];
                 uint32_t nt = (uint32_t)(m.idx.size() / 3);
                 for (uint32_t k = 0; k < nt; ++k) {

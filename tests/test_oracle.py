@@ -344,3 +344,51 @@ def test_camera_look_at_is_orthonormal():
     assert abs(float(np.dot(cam.direction, cam.up))) < 1e-6
     assert abs(float(np.linalg.norm(cam.direction)) - 1) < 1e-6
     assert abs(float(np.linalg.norm(cam.up)) - 1) < 1e-6
+
+
+def _analytic_scene(*surfaces):
+    from voidray_amd.scene import Surfaces  # noqa: F401
+    scene = Scene.empty()
+    m = scene.add_material(Materials.lambertian((0.5, 0.5, 0.5)))
+    for s in surfaces:
+        scene.add_object(m, scene.add_analytic_surface(s))
+    return scene
+
+
+def test_sphere_closed_form():
+    # Sphere::hit (surfaces.rs:46-78): nearest root >= t_min, else the far one
+    from voidray_amd.scene import Surfaces
+    orc = oracle.OracleScene(_analytic_scene(Surfaces.sphere((0.0, 0.0, 0.0), 1.0)))
+    o = np.float32([[0, 5, 0], [0, 0, 0], [0, 5, 0], [3, 0.5, 0], [0, 0, 5]])
+    d = np.float32([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, 1]])
+    r = orc.trace(o, d)
+    assert r["prim"].tolist() == [0, 0, 0xFFFFFFFF, 0, 0xFFFFFFFF]
+    assert r["t"][0] == 4.0                       # outside: near root
+    assert r["t"][1] == 1.0                       # inside: the far root
+    assert abs(r["t"][3] - (3.0 - math.sqrt(0.75))) < 1e-6
+    assert r["u"][0] == 0.0 and r["v"][0] == 0.0  # uv (0, 0)
+
+
+def test_ground_plane_closed_form():
+    # GroundPlane::hit (surfaces.rs:88-104): t = (h - o.y) / d.y, strict bounds, uv = (x, z)
+    from voidray_amd.scene import Surfaces
+    orc = oracle.OracleScene(_analytic_scene(Surfaces.ground_plane(0.5)))
+    s = np.float32(1 / math.sqrt(2))
+    o = np.float32([[1, 2.5, -3], [0, 0.5, 0], [0, 1, 0], [0, 2, 0], [0, 0.5, 0]])
+    d = np.float32([[0, -1, 0], [0, -1, 0], [1, 0, 0], [s, -s, 0], [1, 0, 0]])
+    r = orc.trace(o, d)
+    assert r["prim"].tolist() == [0, 0xFFFFFFFF, 0xFFFFFFFF, 0, 0xFFFFFFFF]
+    assert r["t"][0] == 2.0 and r["u"][0] == 1.0 and r["v"][0] == -3.0
+    # on the plane, moving down: t = 0 <= t_min -> miss; parallel rays miss (t = inf, or 0/0)
+    assert abs(r["t"][3] - 1.5 / s) < 1e-5 and abs(r["u"][3] - 1.5) < 1e-5
+
+
+def test_coincident_spheres_tie_keeps_later_surface():
+    # two identical spheres: equal t; equal centroids keep insertion order in
+    # the median split, so the later surface is the right leaf and wins
+    # (merge_optionals, bvh.rs:164-181)
+    from voidray_amd.scene import Surfaces
+    orc = oracle.OracleScene(_analytic_scene(Surfaces.sphere((0.0, 0.0, 0.0), 1.0),
+                                             Surfaces.sphere((0.0, 0.0, 0.0), 1.0)))
+    r = orc.trace(np.float32([[0, 5, 0], [0.3, 0.2, 4]]), np.float32([[0, -1, 0], [0, 0, -1]]))
+    assert r["prim"].tolist() == [1, 1] and r["t"][0] == 4.0

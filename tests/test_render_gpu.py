@@ -44,6 +44,10 @@ CASES = {
                                                              firefly_clamp=float("inf"))),
     "normal_mode": lambda: (W.material_spheres(dof=False), 40, 40, 2,
                             RenderSettings(total_samples=2, render_mode=RenderMode.Normal)),
+    # analytic spheres and ground plane (surfaces.rs), the reference's spheres example
+    "analytic_spheres_demo": lambda: (W.analytic_spheres(), 64, 64, 4, RenderSettings(total_samples=4, max_bounces=10)),
+    "analytic_normal_mode": lambda: (W.analytic_spheres(), 40, 40, 2,
+                                     RenderSettings(total_samples=2, render_mode=RenderMode.Normal)),
 }
 
 

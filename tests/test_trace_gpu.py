@@ -29,9 +29,15 @@ def _scene_cases():
     o = rng.uniform(0.05, 0.95, (8192, 3)).astype(np.float32)
     d = W._normalize_rows(rng.normal(size=(8192, 3)).astype(np.float32))
     yield "cornell", room, (o, d)
+    # analytic spheres / ground planes mixed with meshes (surfaces.rs)
+    o = rng.uniform(-1.5, 1.5, (8192, 3)).astype(np.float32)
+    d = W._normalize_rows(rng.normal(size=(8192, 3)).astype(np.float32))
+    d[:512, 1] = 0.0
+    d[:512] = W._normalize_rows(d[:512])
+    yield "analytic", W.analytic_and_meshes(), (o, d)
 
 
-@pytest.mark.parametrize("case", ["random4k", "cornell"])
+@pytest.mark.parametrize("case", ["random4k", "cornell", "analytic"])
 def test_closest_hit_matches_oracle(case, gpu_ctx, oracle_lib):
     name, scene, (o, d) = next(c for c in _scene_cases() if c[0] == case)
     ref = oracle_lib.OracleScene(scene).trace(o, d)

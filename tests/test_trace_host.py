@@ -87,3 +87,28 @@ def test_long_needles(oracle_lib):
     d = W._normalize_rows(rng.normal(size=(4096, 3)).astype(np.float32))
     st = _check_all(scene, o, d, oracle_lib, "needles")["stats"]
     assert n <= st["num_tri_records"] <= 8 * n, st
+
+
+def test_analytic_surfaces(oracle_lib):
+    # spheres and ground planes (surfaces.rs) mixed with meshes in the top-level
+    # tree; rays from inside and outside the spheres, grazing the plane
+    scene = W.analytic_and_meshes()
+    rng = np.random.default_rng(8)
+    o = rng.uniform(-1.5, 1.5, (6000, 3)).astype(np.float32)
+    d = W._normalize_rows(rng.normal(size=(6000, 3)).astype(np.float32))
+    d[:500, 1] = 0.0  # parallel to the ground plane
+    d[:500] = W._normalize_rows(d[:500])
+    wide = _check_all(scene, o, d, oracle_lib, "analytic+meshes")
+    assert wide["stats"]["num_triangles"] == scene.num_triangles
+
+
+def test_analytic_spheres_demo(oracle_lib):
+    # the reference's spheres example: every surface analytic, no triangles
+    scene = W.analytic_spheres()
+    rng = np.random.default_rng(9)
+    o = np.tile(np.float32([0.7166, 2.8803, -9.2992]), (4096, 1)) + rng.normal(scale=0.1, size=(4096, 3)).astype(np.float32)
+    d = W._normalize_rows((np.float32([0.8673, 0.9557, 0.2095]) - o + rng.normal(scale=2.0, size=(4096, 3))).astype(np.float32))
+    wide = _check_all(scene, o, d, oracle_lib, "spheres-demo")
+    hits = wide["prim"] != 0xFFFFFFFF
+    assert hits.mean() > 0.5
+    assert wide["stats"]["num_triangles"] == 0

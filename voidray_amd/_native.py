@@ -16,6 +16,8 @@ VR_TEX_RGB32F, VR_TEX_RGBA8 = 0, 1
 VR_SAMPLE_NEAREST, VR_SAMPLE_BILINEAR = 0, 1
 VR_ENV_NONE, VR_ENV_UNIFORM, VR_ENV_HDRI = 0, 1, 2
 VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF = 0, 1
+VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
+VR_SURFACE_ANALYTIC = 0x80000000
 VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
 VR_ENGINE_WIDE, VR_ENGINE_REFERENCE = 0, 1
 VR_MODE_FULL, VR_MODE_NORMAL = 0, 1
@@ -62,13 +64,18 @@ class vr_object(C.Structure):
     _fields_ = [("surface", C.c_uint32), ("material", C.c_uint32)]
 
 
+class vr_analytic(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("center", C.c_float * 3), ("radius", C.c_float), ("height", C.c_float)]
+
+
 class vr_scene_desc(C.Structure):
     _fields_ = [("num_meshes", C.c_uint32), ("meshes", C.POINTER(vr_mesh)),
                 ("num_surfaces", C.c_uint32), ("surface_mesh", C.c_void_p),
                 ("num_objects", C.c_uint32), ("objects", C.POINTER(vr_object)),
                 ("num_materials", C.c_uint32), ("materials", C.POINTER(vr_material)),
                 ("num_textures", C.c_uint32), ("textures", C.POINTER(vr_texture)),
-                ("environment", vr_environment), ("camera", vr_camera), ("build_flags", C.c_uint32)]
+                ("environment", vr_environment), ("camera", vr_camera), ("build_flags", C.c_uint32),
+                ("num_analytic", C.c_uint32), ("analytic", C.POINTER(vr_analytic))]
 
 
 class vr_scene_stats(C.Structure):

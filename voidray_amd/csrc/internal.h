@@ -66,12 +66,23 @@ struct MeshInfo {
 };
 
 struct SurfaceInfo {
-    uint32_t mesh;
+    uint32_t mesh;         // mesh index (analytic surfaces: kNoMesh)
     uint32_t prim_base;    // global prim id of the surface's first triangle
     uint32_t rank_base;    // tie-rank offset (top-level in-order position)
     uint32_t material;     // objects[surface].material (scene.rs:184)
     uint32_t path_off;     // top-level ancestors: indices into top_path
     uint32_t path_len;
+    int32_t analytic;      // index into SceneView::analytic, -1 for a mesh surface
+};
+constexpr uint32_t kNoMesh = 0xFFFFFFFFu;
+
+// Analytic surface (voidray_common/src/surfaces.rs:31-114) with its surface
+// index, tie rank and prim id.
+struct alignas(16) AnalyticDev {
+    int32_t kind;          // VR_ANALYTIC_*
+    float c[3];            // sphere centre
+    float r, h;            // sphere radius, ground-plane height
+    uint32_t surface, rank, prim;
 };
 
 struct GpuMaterial {
@@ -107,6 +118,8 @@ struct SceneView {
     const uint32_t* prim_surface; // global prim -> surface
     const int32_t* ref_parent;    // per ref node: parent ref node, -1 at a mesh root
     const int32_t* prim_anc;      // global prim -> parent of its leaf in the mesh tree (-1: none)
+    const AnalyticDev* analytic;  // analytic surfaces, in surface order
+    uint32_t num_analytic;
     const GpuMaterial* materials;
     const GpuTexture* textures;
     const uint8_t* tex_data;
@@ -169,6 +182,7 @@ struct HostScene {
     std::vector<int32_t> ref_parent, prim_anc, mesh_leaf_parent;
     std::vector<MeshInfo> meshes;
     std::vector<SurfaceInfo> surfaces;
+    std::vector<AnalyticDev> analytic;
     std::vector<float> face_n, vpos, vnrm, vuv, env, env_marg, env_cond, env_pdf;
     std::vector<GpuMaterial> materials;
     std::vector<GpuTexture> textures;

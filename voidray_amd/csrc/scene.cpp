@@ -43,11 +43,21 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     using clk = std::chrono::steady_clock;
     SceneView& v = H.view;
     if (!d) return fail(VR_ERR_INVALID, "null scene description");
-    if (d->num_surfaces > 0 && (!d->surface_mesh || !d->meshes)) return fail(VR_ERR_INVALID, "surfaces without meshes");
+    if (d->num_surfaces > 0 && !d->surface_mesh) return fail(VR_ERR_INVALID, "surfaces without surface_mesh");
     if (d->num_objects < d->num_surfaces)
         return fail(VR_ERR_INVALID, "objects[] must cover every surface index (scene.rs:184 looks up objects[surface])");
-    for (uint32_t i = 0; i < d->num_surfaces; ++i)
-        if (d->surface_mesh[i] >= d->num_meshes) return fail(VR_ERR_INVALID, "surface_mesh index out of range");
+    for (uint32_t i = 0; i < d->num_surfaces; ++i) {
+        const uint32_t sm = d->surface_mesh[i];
+        if (sm & VR_SURFACE_ANALYTIC) {
+            const uint32_t k = sm & ~VR_SURFACE_ANALYTIC;
+            if (!d->analytic || k >= d->num_analytic) return fail(VR_ERR_INVALID, "analytic surface index out of range");
+            const int32_t kind = d->analytic[k].kind;
+            if (kind != VR_ANALYTIC_SPHERE && kind != VR_ANALYTIC_GROUND_PLANE)
+                return fail(VR_ERR_INVALID, "unknown analytic surface kind");
+        } else if (!d->meshes || sm >= d->num_meshes) {
+            return fail(VR_ERR_INVALID, "surface_mesh index out of range");
+        }
+    }
     for (uint32_t i = 0; i < d->num_surfaces; ++i)
         if (d->objects[i].material >= d->num_materials) return fail(VR_ERR_INVALID, "object material out of range");
     for (uint32_t i = 0; i < d->num_materials; ++i) {
@@ -157,7 +167,23 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     // ---- top-level tree over surfaces (scene.rs:72-92, 176) ---------------
     uint32_t S = d->num_surfaces;
     std::vector<Box3> sb(S);
-    for (uint32_t i = 0; i < S; ++i) sb[i] = mesh_bounds[d->surface_mesh[i]];
+    auto is_analytic = [&](uint32_t i) { return (d->surface_mesh[i] & VR_SURFACE_ANALYTIC) != 0; };
+    auto surface_prims = [&](uint32_t i) { return is_analytic(i) ? 1u : minfo[d->surface_mesh[i]].num_tris; };
+    for (uint32_t i = 0; i < S; ++i) {
+        if (!is_analytic(i)) {
+            sb[i] = mesh_bounds[d->surface_mesh[i]];
+            continue;
+        }
+        const vr_analytic& an = d->analytic[d->surface_mesh[i] & ~VR_SURFACE_ANALYTIC];
+        if (an.kind == VR_ANALYTIC_SPHERE) {  // Sphere::bounds (surfaces.rs:36-43)
+            const float r = an.radius;
+            sb[i].lo = mk3(an.center[0] - r, an.center[1] - r, an.center[2] - r);
+            sb[i].hi = mk3(an.center[0] + r, an.center[1] + r, an.center[2] + r);
+        } else {  // GroundPlane::bounds (surfaces.rs:106-113)
+            sb[i].lo = mk3(-kInf, an.height - 0.0001f, -kInf);
+            sb[i].hi = mk3(kInf, an.height + 0.0001f, kInf);
+        }
+    }
     std::vector<RefNode>& top = H.top_nodes;
     std::vector<uint32_t> top_rank;
     build_reference_tree(sb, top, top_rank);
@@ -169,14 +195,29 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t sidx = order[k];
         sinfo[sidx].rank_base = rb;
-        rb += minfo[d->surface_mesh[sidx]].num_tris;
+        rb += surface_prims(sidx);
     }
     uint32_t pb = 0;
+    std::vector<AnalyticDev>& analytic = H.analytic;
     for (uint32_t i = 0; i < S; ++i) {
-        sinfo[i].mesh = d->surface_mesh[i];
+        sinfo[i].mesh = is_analytic(i) ? kNoMesh : d->surface_mesh[i];
         sinfo[i].prim_base = pb;
         sinfo[i].material = d->objects[i].material;
-        pb += minfo[d->surface_mesh[i]].num_tris;
+        sinfo[i].analytic = -1;
+        if (is_analytic(i)) {
+            const vr_analytic& an = d->analytic[d->surface_mesh[i] & ~VR_SURFACE_ANALYTIC];
+            sinfo[i].analytic = (int32_t)analytic.size();
+            AnalyticDev a{};
+            a.kind = an.kind;
+            for (int k = 0; k < 3; ++k) a.c[k] = an.center[k];
+            a.r = an.radius;
+            a.h = an.height;
+            a.surface = i;
+            a.rank = sinfo[i].rank_base;
+            a.prim = pb;
+            analytic.push_back(a);
+        }
+        pb += surface_prims(i);
     }
     uint32_t T = pb;
     // ancestor lists of every surface leaf (root -> parent)
@@ -207,15 +248,21 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     auto t_ref1 = clk::now();
 
     // ---- fast structure: triangle records + SAH quantised BVH4 ------------
-    std::vector<TriRec> recs(T);
+    std::vector<TriRec> recs;
+    recs.reserve(T);
     std::vector<uint32_t>& prim_surface = H.prim_surface;
     prim_surface.assign(T, 0);
     std::vector<int32_t>& prim_anc = H.prim_anc;
     prim_anc.assign(T, -1);
     for (uint32_t i = 0; i < S; ++i) {
+        if (sinfo[i].analytic >= 0) {  // not in the wide BVH: tested per ray (traverse.h)
+            prim_surface[sinfo[i].prim_base] = i;
+            continue;
+        }
         const MeshInfo& mi = minfo[sinfo[i].mesh];
         for (uint32_t t = 0; t < mi.num_tris; ++t) {
-            TriRec& r = recs[sinfo[i].prim_base + t];
+            recs.emplace_back();
+            TriRec& r = recs.back();
             const uint32_t* ix = &midx[3 * (size_t)(mi.tri_base + t)];
             for (int a = 0; a < 3; ++a) {
                 r.p0[a] = vpos[3 * (size_t)(mi.vert_base + ix[0]) + a];
@@ -310,6 +357,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.prim_surface = prim_surface.data();
     v.ref_parent = ref_parent.data();
     v.prim_anc = prim_anc.data();
+    v.analytic = analytic.data();
+    v.num_analytic = (uint32_t)analytic.size();
     v.materials = mats.data();
     v.textures = texs.data();
     v.tex_data = texdata.data();
@@ -319,7 +368,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.env_pdf = env_pdf.data();
 
     vr_scene_stats& st = H.stats;
-    st.num_triangles = T;
+    st.num_triangles = T - (uint32_t)analytic.size();  // mesh triangles
     st.num_surfaces = S;
     st.num_qnodes = (uint32_t)qn.size();
     st.num_ref_nodes = (uint32_t)ref_nodes.size();
@@ -358,6 +407,7 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.prim_surface, &v.prim_surface);
     rc |= upload(s, H.ref_parent, &v.ref_parent);
     rc |= upload(s, H.prim_anc, &v.prim_anc);
+    rc |= upload(s, H.analytic, &v.analytic);
     rc |= upload(s, H.materials, &v.materials);
     rc |= upload(s, H.textures, &v.textures);
     rc |= upload(s, H.tex_data, &v.tex_data);

@@ -135,6 +135,19 @@ VR_HD HitRec make_hit(const SceneView& sv, vf3 o, vf3 d, uint32_t prim, float t,
     HitRec h;
     uint32_t surf = sv.prim_surface[prim];
     SurfaceInfo si = sv.surfaces[surf];
+    if (si.analytic >= 0) {
+        // Sphere::hit / GroundPlane::hit hit records (surfaces.rs:71-77, 96-103):
+        // outward normal (point - centre) / radius or +y; u, v carry the uv
+        const AnalyticDev& a = sv.analytic[si.analytic];
+        h.point = o + d * t;
+        vf3 n = a.kind == VR_ANALYTIC_SPHERE ? (h.point - mk3(a.c[0], a.c[1], a.c[2])) / a.r : mk3(0.f, 1.f, 0.f);
+        h.uv.x = u;
+        h.uv.y = v;
+        h.front = dot3(d, n) < 0.0f;
+        h.normal = h.front ? n : -n;
+        h.surface = surf;
+        return h;
+    }
     MeshInfo mi = sv.meshes[si.mesh];
     uint32_t tri = prim - si.prim_base;
     const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + tri);

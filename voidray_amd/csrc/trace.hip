@@ -119,6 +119,7 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                     vf3 o = mk3(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2]);
                     vf3 d = mk3(a.d[3 * r], a.d[3 * r + 1], a.d[3 * r + 2]);
                     wide_init(st, o, d, a.sv.num_qnodes > 0);
+                    if (a.sv.num_analytic) wide_seed_analytic<ANY>(a.sv, st, a.tmin, a.tmax);
                 }
             }
         }
@@ -256,14 +257,27 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                     RefNode nd = sv.top_nodes[idx];
                     if (nd.obj >= 0) {
                         uint32_t s2 = (uint32_t)nd.obj;
-                        MeshInfo mi = sv.meshes[sv.surfaces[s2].mesh];
-                        if (mi.root >= 0) {
-                            e0 = u32x2{(kCoopMesh << 30) | (uint32_t)mi.root, s2};
-                            np = 1;
+                        const SurfaceInfo& si2 = sv.surfaces[s2];
+                        if (si2.analytic >= 0) {  // analytic surface: its hit, in this lane
+                            float t, u, v;
+                            if (analytic_hit(sv.analytic[si2.analytic], o, d, a.tmin, t, u, v) &&
+                                (t < bt || (t == bt && si2.rank_base > brank))) {
+                                bt = t;
+                                brank = si2.rank_base;
+                                bu = u;
+                                bv = v;
+                                bprim = si2.prim_base;
+                            }
                         } else {
-                            e0 = u32x2{kCoopTri << 30, s2};
-                            np = (int)(mi.num_tris < 4u ? mi.num_tris : 4u);
-                            run = true;
+                            MeshInfo mi = sv.meshes[si2.mesh];
+                            if (mi.root >= 0) {
+                                e0 = u32x2{(kCoopMesh << 30) | (uint32_t)mi.root, s2};
+                                np = 1;
+                            } else {
+                                e0 = u32x2{kCoopTri << 30, s2};
+                                np = (int)(mi.num_tris < 4u ? mi.num_tris : 4u);
+                                run = true;
+                            }
                         }
                     } else {
                         Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};

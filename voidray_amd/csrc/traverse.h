@@ -99,6 +99,73 @@ VR_HD bool confirm_winner(const SceneView& sv, vf3 o, vf3 d, float tmin, float t
 }
 
 
+// ---- analytic surfaces -------------------------------------------------------
+// AnalyticSurface::hit with t_max = inf, as SceneAcceleration::hit passes it:
+// Sphere::hit (voidray_common/src/surfaces.rs:46-78) and GroundPlane::hit
+// (surfaces.rs:88-104), same f32 operations.  Out: t and the hit record's uv.
+// A ray lying exactly in a ground plane (t = 0/0) counts as a miss.
+VR_HD bool analytic_hit(const AnalyticDev& a, vf3 o, vf3 d, float tmin, float& t, float& u, float& v) {
+    if (a.kind == VR_ANALYTIC_SPHERE) {
+        const vf3 oc = o - mk3(a.c[0], a.c[1], a.c[2]);
+        const float aa = mag2(d);
+        const float half_b = dot3(oc, d);
+        const float c = mag2(oc) - a.r * a.r;
+        const float disc = half_b * half_b - aa * c;
+        if (disc < 0.0f) return false;
+        const float sqrtd = sqrtf(disc);
+        float root = (-half_b - sqrtd) / aa;
+        if (root < tmin || kInf < root) {
+            root = (-half_b + sqrtd) / aa;
+            if (root < tmin || kInf < root) return false;
+        }
+        t = root;
+        u = v = 0.0f;
+        return true;
+    }
+    const float tt = (a.h - o.y) / d.y;
+    if (tt <= tmin || tt >= kInf || tt != tt) return false;
+    const vf3 p = o + d * tt;  // Ray::at
+    t = tt;
+    u = p.x;
+    v = p.z;
+    return true;
+}
+
+// Every top-level ancestor of `surf` passes the reference slab test.
+VR_HD bool top_path_pass(const SceneView& sv, uint32_t surf, vf3 o, vf3 d, float tmin) {
+    const SurfaceInfo& si = sv.surfaces[surf];
+    for (uint32_t k = 0; k < si.path_len; ++k) {
+        const RefNode& nd = sv.top_nodes[sv.top_path[si.path_off + k]];
+        Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
+        if (!ref_slab_hit(b, o, d, tmin, kInf)) return false;
+    }
+    return true;
+}
+
+// Closest analytic surface the reference reaches (lexicographic min of (t,
+// -rank), like the triangles); with `first_below` set, the first one found
+// with t < first_below.  Returns its index or -1.
+VR_HD int analytic_best(const SceneView& sv, vf3 o, vf3 d, float tmin, float first_below, float& bt, uint32_t& brank,
+                        float& bu, float& bv) {
+    int best = -1;
+    for (uint32_t k = 0; k < sv.num_analytic; ++k) {
+        const AnalyticDev& a = sv.analytic[k];
+        float t, u, v;
+        if (!analytic_hit(a, o, d, tmin, t, u, v)) continue;
+        if (best >= 0 && !(t < bt || (t == bt && a.rank > brank))) continue;
+        if (!top_path_pass(sv, a.surface, o, d, tmin)) continue;
+        best = (int)k;
+        bt = t;
+        brank = a.rank;
+        bu = u;
+        bv = v;
+        if (t < first_below) break;
+    }
+    return best;
+}
+
+constexpr uint32_t kAnalyticSlot = 0x40000000u;  // best_slot of an analytic winner (triangle slots < 2^27)
+
 // ---- wide engine as a resumable per-lane state machine ----------------------
 // The stack lives in LDS (LDS_STACK entries per lane, strided by `stride`) and
 // spills its oldest entries to a per-lane global region (`spill`, strided by
@@ -141,6 +208,28 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     st.have = nonempty;
     st.unconfirmed = false;
     st.found = false;
+}
+
+// Analytic surfaces are few and not in the wide BVH: each ray tests them all
+// up front, and the best one seeds the closest hit (so the triangle walk culls
+// against it and ties resolve by rank as usual).  An any-hit ray that hits one
+// below tmax_any is finished.
+template <bool ANY>
+VR_HD void wide_seed_analytic(const SceneView& sv, WideState& st, float tmin, float tmax_any) {
+    float t = kInf, u = 0.f, v = 0.f;
+    uint32_t rank = 0;
+    const int k = analytic_best(sv, st.o, st.d, tmin, ANY ? tmax_any : -kInf, t, rank, u, v);
+    if (k < 0) return;
+    if (ANY) {
+        if (!(t < tmax_any)) return;
+        st.found = true;
+        st.have = false;
+    }
+    st.tbest = t;
+    st.best_rank = rank;
+    st.best_slot = kAnalyticSlot | (uint32_t)k;
+    st.bu = u;
+    st.bv = v;
 }
 
 // Slot of stack entry i in the LDS ring (i >= 0).
@@ -318,12 +407,20 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
             out.t = st.tbest;
             out.u = st.bu;
             out.v = st.bv;
-            out.prim = tri_word(sv.tris, st.best_slot, 9);
+            out.prim = (st.best_slot & kAnalyticSlot) ? sv.analytic[st.best_slot & ~kAnalyticSlot].prim
+                                                      : tri_word(sv.tris, st.best_slot, 9);
             return kStepDone;
         }
         return st.unconfirmed ? kStepFallback : kStepDone;
     }
     if (st.best_slot == 0xFFFFFFFFu) return kStepDone;
+    if (st.best_slot & kAnalyticSlot) {  // confirmed when it was seeded
+        out.t = st.tbest;
+        out.u = st.bu;
+        out.v = st.bv;
+        out.prim = sv.analytic[st.best_slot & ~kAnalyticSlot].prim;
+        return kStepDone;
+    }
     vf3 p0, p1, p2;
     load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
     if (!confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
@@ -342,6 +439,7 @@ VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_a
                      HitOut& out) {
     WideState st;
     wide_init(st, o, d, sv.num_qnodes > 0);
+    if (sv.num_analytic) wide_seed_analytic<ANY>(sv, st, tmin, tmax_any);
     int rc;
     while ((rc = wide_step<LDS_STACK, ANY>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
     }
@@ -378,6 +476,20 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
         }
         uint32_t surf = (uint32_t)tn.obj;
         SurfaceInfo si = sv.surfaces[surf];
+        if (si.analytic >= 0) {
+            float t, u, v;
+            if (analytic_hit(sv.analytic[si.analytic], o, d, tmin, t, u, v) &&
+                (!have || t < bt || (t == bt && si.rank_base > brank))) {
+                have = true;
+                bt = t;
+                brank = si.rank_base;
+                out.t = t;
+                out.u = u;
+                out.v = v;
+                out.prim = si.prim_base;
+            }
+            continue;
+        }
         MeshInfo mi = sv.meshes[si.mesh];
         auto test = [&](uint32_t tri) {
             const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + tri);

@@ -117,7 +117,26 @@ class Mesh:
         return len(self.indices)
 
 
+@dataclass
+class AnalyticSurface:
+    """Sphere (surfaces.rs:31-79) or GroundPlane (surfaces.rs:81-114), f32 parameters."""
+    kind: int
+    center: tuple = (0.0, 0.0, 0.0)
+    radius: float = 0.0
+    height: float = 0.0
+
+
 class Surfaces:
+    @staticmethod
+    def sphere(center, radius):
+        """surfaces.rs:10-12."""
+        return AnalyticSurface(N.VR_ANALYTIC_SPHERE, tuple(float(x) for x in _f3(center)), float(f32(radius)))
+
+    @staticmethod
+    def ground_plane(height):
+        """surfaces.rs:14-16."""
+        return AnalyticSurface(N.VR_ANALYTIC_GROUND_PLANE, height=float(f32(height)))
+
     @staticmethod
     def quad(q1, q2, q3, q4):
         """surfaces.rs:18-28: Vertex::position vertices, indices [0,1,2, 2,0,3]."""
@@ -238,8 +257,9 @@ class Scene:
                                      float(f32(f32(np.pi) / f32(6.0))))
         self.materials: List[Material] = []
         self.objects: List[tuple] = []
-        self.surfaces: List[int] = []  # surface -> mesh index
+        self.surfaces: List[int] = []  # surface -> mesh index, or VR_SURFACE_ANALYTIC | analytic index
         self.meshes: List[Mesh] = []
+        self.analytic: List[AnalyticSurface] = []
         self.textures: List[Texture] = []
         self.environment: Optional[Environment] = None
 
@@ -256,6 +276,12 @@ class Scene:
         self.surfaces.append(len(self.meshes) - 1)
         return len(self.surfaces) - 1
 
+    def add_analytic_surface(self, surface):
+        """scene.rs:118-124."""
+        self.analytic.append(surface)
+        self.surfaces.append(N.VR_SURFACE_ANALYTIC | (len(self.analytic) - 1))
+        return len(self.surfaces) - 1
+
     def add_object(self, material, surface):
         self.objects.append((int(surface), int(material)))
         return len(self.objects) - 1
@@ -266,7 +292,7 @@ class Scene:
 
     @property
     def num_triangles(self):
-        return sum(self.meshes[m].num_triangles for m in self.surfaces)
+        return sum(self.meshes[m].num_triangles for m in self.surfaces if not m & N.VR_SURFACE_ANALYTIC)
 
     def to_desc(self, build_flags=0):
         """Flat C description; returns (desc, keepalive)."""
@@ -312,9 +338,13 @@ class Scene:
             cam.has_dof = 1
             cam.aperture = c.dof[0]
             cam.focal_point = N.F3(*_f3(c.dof[1]))
+        ana = (N.vr_analytic * max(1, len(self.analytic)))()
+        for i, a in enumerate(self.analytic):
+            ana[i] = N.vr_analytic(a.kind, N.F3(*a.center), a.radius, a.height)
         desc = N.vr_scene_desc(len(self.meshes), meshes, len(self.surfaces), ptr(surf), len(self.objects), objs,
-                               len(self.materials), mats, len(self.textures), texs, env, cam, build_flags)
-        keep += [meshes, objs, mats, texs]
+                               len(self.materials), mats, len(self.textures), texs, env, cam, build_flags,
+                               len(self.analytic), ana)
+        keep += [meshes, objs, mats, texs, ana]
         return desc, keep
 
     def build_acceleration(self, ctx=None, build_flags=0):

@@ -407,3 +407,62 @@ def serving_requests(n=64, pool_size=8, seed=7):
         w, h = SERVING_RESOLUTIONS[int(rng.integers(len(SERVING_RESOLUTIONS)))]
         out.append((int(rng.integers(pool_size)), w, h, int(rng.integers(1, 33)), seed + i))
     return out
+
+
+def _hex(c):
+    """hex_color (util/color.rs): 0xRRGGBB -> RGB in [0, 1]."""
+    return ((c >> 16) & 255) / 255.0, ((c >> 8) & 255) / 255.0, (c & 255) / 255.0
+
+
+def analytic_spheres(tex_size=64, sky=(64, 32)):
+    """The reference's spheres example (voidray_app/src/examples/spheres.rs:5-60):
+    analytic spheres (metals, glass, a glass shell around a diffuse core, a red
+    emissive sphere) over an analytic ground plane with albedo and normal
+    textures, DOF camera, HDRI.  The image assets (wood .tif, indoor .exr) are
+    replaced by procedural textures and sky of the same roles."""
+    from .scene import Surfaces
+    scene = Scene.empty()
+    red = scene.add_material(Materials.metal(_hex(0xE78999), 0.05))
+    yellow = scene.add_material(Materials.dielectric(1.5))
+    green = scene.add_material(Materials.metal(_hex(0xB3E7AA), 0.1))
+    blue = scene.add_material(Materials.metal(_hex(0x7CA3E7), 0.01))
+    grey = scene.add_material(Materials.metal(_hex(0xAAAAAA), 0.01))
+    diffuse = scene.add_material(Materials.lambertian(_hex(0x7CA3E7)))
+    light_mtl = scene.add_material(Materials.colored_emissive(_hex(0xFF0F0F), 200.0))
+    for pos, mtl in (((0.5, 1.0, 4.0), red), ((3.15, 1.5, -0.7), yellow), ((0.6, 0.6, -2.0), green),
+                     ((-1.7, 1.1, -0.2), blue), ((1.2, 0.5, 0.4), grey)):
+        scene.add_object(mtl, scene.add_analytic_surface(Surfaces.sphere(pos, pos[1])))
+    glass_2 = scene.add_material(Materials.dielectric(1.5))
+    sph = scene.add_analytic_surface(Surfaces.sphere((1.5, 0.7, -3.1), 0.7))
+    sph_inner = scene.add_analytic_surface(Surfaces.sphere((1.5, 0.695, -3.1), 0.695))
+    scene.add_object(glass_2, sph)
+    scene.add_object(diffuse, sph_inner)
+    scene.add_object(light_mtl, scene.add_analytic_surface(Surfaces.sphere((1.2, 8.0, -1.5), 2.0)))
+    scene.camera = Camera.look_at((0.7166, 2.8803, -9.2992), (0.8673, 0.9557, 0.2095), (0.0, 1.0, 0.0), 0.6911)
+    scene.camera.dof = (0.12, (0.1, 0.6, -2.0))
+    alb = scene.add_image_texture(procedural_texture(tex_size, 21, "albedo"), SampleType.Bilinear)
+    nrm = scene.add_image_texture(procedural_texture(tex_size, 22, "normal"), SampleType.Nearest)
+    gnd = scene.add_analytic_surface(Surfaces.ground_plane(0.0))
+    scene.add_object(scene.add_material(Materials.lambertian_texture(alb, nrm)), gnd)
+    scene.environment = Environments.hdri(procedural_sky(*sky, sun_peak=50.0, sun_sharpness=40.0))
+    return scene
+
+
+def analytic_and_meshes(seed=13):
+    """Analytic spheres and a ground plane interleaved with triangle meshes in
+    the surface list (so the top-level tree mixes both), including a sphere
+    coinciding with a UV-sphere mesh (equal-t ties across surface kinds)."""
+    from .scene import Surfaces
+    rng = _rng(seed)
+    scene = Scene.empty()
+    mat = scene.add_material(Materials.lambertian((0.6, 0.6, 0.6)))
+    scene.add_object(mat, scene.add_analytic_surface(Surfaces.ground_plane(-0.25)))
+    for i in range(6):
+        c = tuple(float(x) for x in rng.uniform(-1, 1, 3))
+        if i % 2:
+            scene.add_object(mat, scene.add_mesh(uv_sphere(c, 0.3, 16, 8)))
+        else:
+            scene.add_object(mat, scene.add_analytic_surface(Surfaces.sphere(c, 0.3)))
+    scene.add_object(mat, scene.add_mesh(random_triangles(2000, seed)))
+    scene.add_object(mat, scene.add_analytic_surface(Surfaces.sphere((0.0, 2.0, 0.0), 0.5)))
+    return scene
