@@ -49,6 +49,7 @@ typedef struct vr_mesh {
 #define VR_MAT_DIELECTRIC 2      /* Dielectric::scatter  simple.rs:200-232 */
 #define VR_MAT_EMISSION 3        /* Emission::scatter    simple.rs:175-185 */
 #define VR_MAT_LAMBERTIAN_BSDF 4 /* LambertianBSDF via traits.rs:19-41      */
+#define VR_MAT_MICROFACET 5      /* MicrofacetBSDF (voidray_common/src/microfacet.rs) via traits.rs:19-41 */
 
 typedef struct vr_material {
     int32_t kind;
@@ -56,6 +57,10 @@ typedef struct vr_material {
     float param;            /* Metal fuzz, Dielectric ir                  */
     int32_t albedo_texture; /* Lambertian texture albedo, -1 = none       */
     int32_t normal_texture; /* Lambertian normal map, -1 = none           */
+    /* MicrofacetBSDF (microfacet.rs:9-27); emittance is stored but, as in the
+     * reference's BSDFMaterial::scatter, never emitted */
+    float index, roughness, metallic, emittance;
+    int32_t transparent;
 } vr_material;
 
 #define VR_TEX_RGB32F 0   /* image.to_rgb32f() layout (texture.rs:37)   */

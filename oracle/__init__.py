@@ -39,6 +39,8 @@ def lib():
         L.orc_philox4x32_10.argtypes = [_P, _P, _P]
         L.orc_path_words.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
         L.orc_unit_sphere_f32.argtypes = [C.c_uint64, C.c_uint32, _P]
+        L.orc_microfacet_bsdf_f64.argtypes = [_P, _P, _P, _P, _P]
+        L.orc_microfacet_sample_f64.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, _P, _P]
         for nm in ("orc_scene_new_f32", "orc_scene_new_f64"):
             getattr(L, nm).restype = _P
             getattr(L, nm).argtypes = [C.POINTER(N.vr_scene_desc)]
@@ -147,6 +149,33 @@ def unit_sphere_samples(seed, count):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def _mf_params(index, roughness, metallic, transparent, color):
+    return np.array([index, roughness, metallic, 1.0 if transparent else 0.0, *color], np.float64)
+
+
+def microfacet_bsdf(n, wo, wi, index=1.5, roughness=1.0, metallic=0.0, transparent=False, color=(1, 1, 1)):
+    """MicrofacetBSDF::bsdf (microfacet.rs:120-213) in FP64."""
+    m = _mf_params(index, roughness, metallic, transparent, color)
+    a = [np.ascontiguousarray(x, np.float64) for x in (n, wo, wi)]
+    out = np.zeros(3, np.float64)
+    lib().orc_microfacet_bsdf_f64(m.ctypes.data, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                  out.ctypes.data)
+    return out
+
+
+def microfacet_sample(n, wo, count, seed=1, index=1.5, roughness=1.0, metallic=0.0, transparent=False,
+                      color=(1, 1, 1)):
+    """MicrofacetBSDF::sample (microfacet.rs:215-312) in FP64: (wi [count,3], pdf [count]; 0 = None)."""
+    m = _mf_params(index, roughness, metallic, transparent, color)
+    nn = np.ascontiguousarray(n, np.float64)
+    ww = np.ascontiguousarray(wo, np.float64)
+    wi = np.zeros((count, 3), np.float64)
+    pdf = np.zeros(count, np.float64)
+    lib().orc_microfacet_sample_f64(m.ctypes.data, nn.ctypes.data, ww.ctypes.data, seed, count, wi.ctypes.data,
+                                    pdf.ctypes.data)
+    return wi, pdf
 
 
 class OracleScene:

@@ -162,6 +162,8 @@ template <class R> struct Material {
     V3<R> color;
     R param;
     int albedo_tex, normal_tex;
+    R index = R(1.5), roughness = R(1), metallic = R(0);  // MicrofacetBSDF (microfacet.rs:9-27)
+    bool transparent = false;
 };
 
 template <class R> struct Scene {
@@ -256,7 +258,8 @@ template <class R> Scene<R>* make_scene(const vr_scene_desc* d) {
     s->objects.assign(d->objects, d->objects + d->num_objects);
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const vr_material& m = d->materials[i];
-        s->materials.push_back(Material<R>{m.kind, load3<R>(m.color, 0), R(m.param), m.albedo_texture, m.normal_texture});
+        s->materials.push_back(Material<R>{m.kind, load3<R>(m.color, 0), R(m.param), m.albedo_texture, m.normal_texture,
+                                           R(m.index), R(m.roughness), R(m.metallic), m.transparent != 0});
     }
     s->textures.resize(d->num_textures);
     for (uint32_t i = 0; i < d->num_textures; ++i) make_texture(s->textures[i], d->textures[i]);
@@ -552,6 +555,133 @@ template <class R> bool near_zero(V3<R> v) {
 }
 // reflect / refract (math.rs:12-21)
 template <class R> V3<R> reflect(V3<R> v, V3<R> n) { return v - (R(2) * dot(v, n)) * n; }
+
+// ---- MicrofacetBSDF (voidray_common/src/microfacet.rs) ----------------------
+// powi as LLVM expands it: x^3 = x * x^2, x^5 = x * (x^2)^2.
+template <class R> R powi2(R x) { return x * x; }
+template <class R> R powi3(R x) { return x * (x * x); }
+template <class R> R powi5(R x) {
+    R x2 = x * x;
+    return x * (x2 * x2);
+}
+// local_to_world(n) * h (util/math.rs:50-59): the matrix is filled column by
+// column with (ns.x, nss.x, n.x), ..., so the product is (ns.h, nss.h, n.h).
+template <class R> V3<R> local_to_world_mul(V3<R> n, V3<R> h) {
+    V3<R> ns = std::isnormal(n.x) ? normalize(v3<R>(n.y, -n.x, R(0))) : normalize(v3<R>(R(0), -n.z, n.y));
+    V3<R> nss = cross(n, ns);
+    return v3<R>(dot(ns, h), dot(nss, h), dot(n, h));
+}
+// UnitCircle (rand_distr 0.4.3): von Neumann's rejection method.
+template <class R> V2<R> unit_circle(PathRng& rng, uint64_t& capped) {
+    for (int i = 0; i < kRejectCap; ++i) {
+        R x1 = R(rng.uniform() * 2.0f - 1.0f);
+        R x2 = R(rng.uniform() * 2.0f - 1.0f);
+        R sum = x1 * x1 + x2 * x2;
+        if (sum >= R(1)) continue;
+        R diff = x1 * x1 - x2 * x2;
+        return V2<R>{diff / sum, (R(2) * x1) * x2 / sum};
+    }
+    ++capped;
+    return V2<R>{R(1), R(0)};
+}
+template <class R> R beckmann_d(R m2, R nh2) {
+    return std::exp((nh2 - R(1)) / (m2 * nh2)) / (((m2 * pi_v<R>()) * nh2) * nh2);
+}
+template <class R> bool sign_positive(R x) { return !std::signbit(x); }
+
+// MicrofacetBSDF::bsdf (microfacet.rs:120-213)
+template <class R> V3<R> microfacet_bsdf(const Material<R>& m, V3<R> n, V3<R> wo, V3<R> wi) {
+    const V3<R> one = v3<R>(R(1), R(1), R(1));
+    R n_dot_wi = dot(n, wi), n_dot_wo = dot(n, wo);
+    bool wi_outside = sign_positive(n_dot_wi), wo_outside = sign_positive(n_dot_wo);
+    if (!m.transparent && (!wi_outside || !wo_outside)) return v3<R>(0, 0, 0);
+    R m2 = m.roughness * m.roughness;
+    R f0s = powi2((m.index - R(1)) / (m.index + R(1)));
+    V3<R> f0 = v3<R>(f0s, f0s, f0s) * (R(1) - m.metallic) + m.color * m.metallic;  // lerp (math.rs:40-45)
+    if (wi_outside == wo_outside) {
+        V3<R> h = normalize(wi + wo);
+        R wo_dot_h = dot(wo, h), n_dot_h = dot(n, h);
+        R nh2 = powi2(n_dot_h);
+        R d = beckmann_d(m2, nh2);
+        V3<R> f = (!wi_outside && std::sqrt(R(1) - wo_dot_h * wo_dot_h) * m.index > R(1))
+                      ? one
+                      : f0 + (one - f0) * powi5(R(1) - wo_dot_h);
+        R g = rmin(n_dot_wi * n_dot_h, n_dot_wo * n_dot_h);
+        g = (R(2) * g) / wo_dot_h;
+        g = rmin(g, R(1));
+        V3<R> specular = ((d * f) * g) / ((R(4) * n_dot_wo) * n_dot_wi);
+        if (m.transparent) return specular;
+        V3<R> diffuse = mulv(one - f, m.color) / pi_v<R>();
+        return specular + diffuse;
+    }
+    R eta_t = wo_outside ? m.index : R(1) / m.index;
+    V3<R> h = normalize(wi * eta_t + wo);
+    R wi_dot_h = dot(wi, h), wo_dot_h = dot(wo, h), n_dot_h = dot(n, h);
+    R nh2 = powi2(n_dot_h);
+    R d = beckmann_d(m2, nh2);
+    V3<R> f = f0 + (one - f0) * powi5(R(1) - std::fabs(wi_dot_h));
+    R g = rmin(std::fabs(n_dot_wi * n_dot_h), std::fabs(n_dot_wo * n_dot_h));
+    g = (R(2) * g) / std::fabs(wo_dot_h);
+    g = rmin(g, R(1));
+    R a = std::fabs((wi_dot_h * wo_dot_h) / (n_dot_wi * n_dot_wo));
+    V3<R> btdf = a * (((d * (one - f)) * g) / powi2(eta_t * wi_dot_h + wo_dot_h));
+    return mulv(btdf, m.color);
+}
+
+template <class R> R beckmann_pdf(R m2, V3<R> n, V3<R> h) {
+    R cos_t = std::fabs(dot(h, n));
+    R sin_t = std::sqrt(R(1) - cos_t * cos_t);
+    return (R(1) / ((pi_v<R>() * m2) * powi3(cos_t))) * std::exp((-powi2(sin_t / cos_t)) / m2);
+}
+
+// MicrofacetBSDF::sample (microfacet.rs:215-312); false = None.
+template <class R>
+bool microfacet_sample(const Material<R>& m, V3<R> n, V3<R> wo, PathRng& rng, uint64_t& capped, V3<R>& wi, R& pdf) {
+    R m2 = m.roughness * m.roughness;
+    R f0 = powi2((m.index - R(1)) / (m.index + R(1)));
+    R f = (R(1) - m.metallic) * f0 + m.metallic * (((m.color.x + m.color.y) + m.color.z) / R(3));
+    f = f * (R(1) - R(0.2f)) + R(1) * R(0.2f);
+    R eta_t = dot(wo, n) > R(0) ? m.index : R(1) / m.index;
+    auto beckmann = [&]() {
+        R theta = std::atan(std::sqrt(m2 * -std::log(R(rng.uniform()))));
+        R sin_t = std::sin(theta), cos_t = std::cos(theta);
+        V2<R> c = unit_circle<R>(rng, capped);
+        return local_to_world_mul(n, v3<R>(c.x * sin_t, c.y * sin_t, cos_t));
+    };
+    if (R(rng.uniform()) < f) {
+        wi = -reflect(wo, beckmann());
+    } else if (!m.transparent) {
+        V2<R> q = unit_disc<R>(rng, capped);
+        R z = std::sqrt((R(1) - q.x * q.x) - q.y * q.y);
+        wi = local_to_world_mul(n, v3<R>(q.x, q.y, z));
+    } else {
+        V3<R> h = beckmann();
+        R cos_to = dot(h, wo);
+        V3<R> wo_perp = wo - h * cos_to;
+        V3<R> wi_perp = (-wo_perp) / eta_t;
+        R sin2_ti = dot(wi_perp, wi_perp);
+        if (sin2_ti > R(1)) return false;
+        R cos_ti = std::sqrt(R(1) - sin2_ti);
+        wi = ((-std::copysign(R(1), cos_to)) * cos_ti) * h + wi_perp;
+    }
+    R p = R(0);
+    {
+        V3<R> h = normalize(wi + wo);
+        p += f * beckmann_pdf(m2, n, h) / (R(4) * std::fabs(dot(h, wo)));
+    }
+    if (!m.transparent) {
+        p += (R(1) - f) * rmax(dot(wi, n), R(0)) / pi_v<R>();
+    } else if (sign_positive(dot(wo, n)) != sign_positive(dot(wi, n))) {
+        V3<R> h = normalize(wi * eta_t + wo);
+        R p_h = beckmann_pdf(m2, n, h);
+        R h_dot_wo = dot(h, wo), h_dot_wi = dot(h, wi);
+        R jacobian = std::fabs(h_dot_wo) / powi2(eta_t * h_dot_wi + h_dot_wo);
+        p += ((R(1) - f) * p_h) * jacobian;
+    }
+    if (p == R(0)) return false;
+    pdf = p;
+    return true;
+}
 template <class R> V3<R> refract(V3<R> uv, V3<R> n, R eta) {
     R cos_t = rmin(dot(n, -uv), R(1));
     V3<R> perp = eta * (uv + cos_t * n);
@@ -630,6 +760,22 @@ Scatter<R> scatter(const Scene<R>& s, const Material<R>& m, const Ray<R>& ray, c
             out.att = m.color;
             out.has_ray = false;
             return out;
+        case VR_MAT_MICROFACET: {
+            // blanket impl (traits.rs:19-41) + MicrofacetBSDF: wo = the ray's direction, normalised
+            V3<R> wo = normalize(ray.d);
+            V3<R> wi;
+            R pdf;
+            if (!microfacet_sample(m, hit.normal, wo, rng, rc.capped, wi, pdf)) {
+                out.att = v3<R>(0, 0, 0);
+                out.has_ray = false;
+                return out;
+            }
+            V3<R> f = microfacet_bsdf(m, hit.normal, wo, wi);
+            out.ray = ray_new(ray.at(hit.t), wi);
+            out.att = (f * std::fabs(dot(wi, hit.normal))) * (R(1) / pdf);
+            out.has_ray = true;
+            return out;
+        }
         case VR_MAT_LAMBERTIAN_BSDF: {
             // blanket impl (traits.rs:19-41) + LambertianBSDF (simple.rs:60-81)
             if (magnitude2(hit.normal) == R(0)) {  // the reference asserts here
@@ -1006,6 +1152,46 @@ void orc_unit_sphere_f32(uint64_t seed, uint32_t count, float* out) {
         out[3 * i] = p.x;
         out[3 * i + 1] = p.y;
         out[3 * i + 2] = p.z;
+    }
+}
+
+// MicrofacetBSDF pieces for the oracle's own tests.  mat: index, roughness,
+// metallic, transparent, r, g, b.
+static Material<double> microfacet_material(const double* mat) {
+    Material<double> m{};
+    m.kind = VR_MAT_MICROFACET;
+    m.index = mat[0];
+    m.roughness = mat[1];
+    m.metallic = mat[2];
+    m.transparent = mat[3] != 0.0;
+    m.color = v3<double>(mat[4], mat[5], mat[6]);
+    return m;
+}
+void orc_microfacet_bsdf_f64(const double* mat, const double* n, const double* wo, const double* wi, double* out) {
+    Material<double> m = microfacet_material(mat);
+    V3<double> f = microfacet_bsdf(m, v3<double>(n[0], n[1], n[2]), v3<double>(wo[0], wo[1], wo[2]),
+                                   v3<double>(wi[0], wi[1], wi[2]));
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+}
+// count samples (stream: pixel i of the given seed); pdf[i] = 0 for None
+void orc_microfacet_sample_f64(const double* mat, const double* n, const double* wo, uint64_t seed, uint32_t count,
+                               double* wi, double* pdf) {
+    Material<double> m = microfacet_material(mat);
+    uint64_t capped = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        PathRng r(seed, 0, i, 0);
+        V3<double> w;
+        double p = 0.0;
+        if (!microfacet_sample(m, v3<double>(n[0], n[1], n[2]), v3<double>(wo[0], wo[1], wo[2]), r, capped, w, p)) {
+            w = v3<double>(0, 0, 0);
+            p = 0.0;
+        }
+        wi[3 * i] = w.x;
+        wi[3 * i + 1] = w.y;
+        wi[3 * i + 2] = w.z;
+        pdf[i] = p;
     }
 }
 

@@ -22,18 +22,35 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(N.SIGNATURES), set(names) ^ set(N.SIGNATURES)
 
 
-def test_struct_sizes_match_header_layout():
+def test_struct_sizes_match_header_layout(tmp_path):
+    # sizes and field offsets as the C compiler lays out include/voidray_hip.h,
+    # against the ctypes mirror
     import ctypes as C
-    assert C.sizeof(N.vr_mesh) == 8 + 4 * 8
-    assert C.sizeof(N.vr_material) == 4 + 12 + 4 + 4 + 4
-    assert C.sizeof(N.vr_trace_desc) == 16
-    assert C.sizeof(N.vr_render_desc) == 4 * 10 + 8 + 8
-    assert C.sizeof(N.vr_request) == 8 + C.sizeof(N.vr_render_desc) + 8
-    assert C.sizeof(N.vr_tuning) == 8
-    assert C.sizeof(N.vr_cache_desc) == 16 + 24 + 20 + 4 + 8
-    assert C.sizeof(N.vr_cache_info) == 16 + 64 + 16 + 16
-    assert C.sizeof(N.vr_cache_frame) == 24
-    assert C.sizeof(N.vr_cache_stats) == 40
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        import pytest
+        pytest.skip("gcc not available")
+    structs = [n for n in dir(N) if n.startswith("vr_") and isinstance(getattr(N, n), type)
+               and issubclass(getattr(N, n), C.Structure)]
+    assert len(structs) >= 14
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "voidray_hip.h"', "int main(void) {"]
+    for n in structs:
+        lines.append(f'printf("{n} %zu\\n", sizeof({n}));')
+        for f, _ in getattr(N, n)._fields_:
+            lines.append(f'printf("{n}.{f} %zu\\n", offsetof({n}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for n in structs:
+        t = getattr(N, n)
+        assert C.sizeof(t) == int(out[n]), n
+        for f, _ in t._fields_:
+            assert getattr(t, f).offset == int(out[f"{n}.{f}"]), f"{n}.{f}"
 
 
 def test_errors_do_not_unwind():

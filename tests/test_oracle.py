@@ -392,3 +392,40 @@ def test_coincident_spheres_tie_keeps_later_surface():
                                              Surfaces.sphere((0.0, 0.0, 0.0), 1.0)))
     r = orc.trace(np.float32([[0, 5, 0], [0.3, 0.2, 4]]), np.float32([[0, -1, 0], [0, 0, -1]]))
     assert r["prim"].tolist() == [1, 1] and r["t"][0] == 4.0
+
+
+def test_microfacet_bsdf_closed_form():
+    # MicrofacetBSDF::bsdf (microfacet.rs:120-213) at normal incidence:
+    # h = n, D = 1/(pi m^2), Schlick F = F0 = ((ior-1)/(ior+1))^2, G = 1:
+    # Cook-Torrance specular F0 / (4 pi m^2) plus Lambert (1 - F0) c / pi
+    z = np.array([0.0, 0.0, 1.0])
+    c = np.array([0.8, 0.5, 0.2])
+    f0 = (0.5 / 2.5) ** 2
+    for m in (0.5, 0.2):
+        f = oracle.microfacet_bsdf(z, z, z, index=1.5, roughness=m, color=c)
+        expect = f0 / (4 * math.pi * m * m) + (1 - f0) * c / math.pi
+        assert np.allclose(f, expect, rtol=1e-12)
+    # metallic = 1: F0 = colour, no diffuse energy left at normal incidence
+    f = oracle.microfacet_bsdf(z, z, z, index=1.5, roughness=0.5, metallic=1.0, color=c)
+    assert np.allclose(f, c / (4 * math.pi * 0.25) + (1 - c) * c / math.pi, rtol=1e-12)
+    # opaque: nothing below the surface
+    assert np.all(oracle.microfacet_bsdf(z, z, -z, roughness=0.5) == 0)
+    # straight-through transmission (documented BTDF, eta_o = 1, eta_i = 1.5):
+    # h = -n, |h.wi||h.wo|/(|n.wi||n.wo|) = 1, (1 - F0) D G / (1.5 - 1)^2 = 4 D (1 - F0)
+    f = oracle.microfacet_bsdf(z, z, -z, index=1.5, roughness=0.5, transparent=True, color=c)
+    assert np.allclose(f, 4 * (1 / (math.pi * 0.25)) * (1 - f0) * c, rtol=1e-12)
+
+
+def test_microfacet_sampling_density():
+    # MicrofacetBSDF::sample (microfacet.rs:215-312), opaque, normal +z (where
+    # the reference's local_to_world is a rotation about the normal): the
+    # returned pdf is the density of the sampled directions, so
+    # E[1{wi.n > 0} / pdf] = 2 pi
+    z = np.array([0.0, 0.0, 1.0])
+    wo = np.array([0.3, 0.1, 0.9])
+    wo /= np.linalg.norm(wo)
+    for rough in (0.3, 0.7):
+        wi, pdf = oracle.microfacet_sample(z, wo, 200000, roughness=rough, color=(0.6, 0.6, 0.6))
+        assert np.all(pdf > 0)
+        est = np.mean((wi[:, 2] > 0) / pdf)
+        assert abs(est / (2 * math.pi) - 1) < 0.02, (rough, est)

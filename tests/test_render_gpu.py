@@ -48,6 +48,8 @@ CASES = {
     "analytic_spheres_demo": lambda: (W.analytic_spheres(), 64, 64, 4, RenderSettings(total_samples=4, max_bounces=10)),
     "analytic_normal_mode": lambda: (W.analytic_spheres(), 40, 40, 2,
                                      RenderSettings(total_samples=2, render_mode=RenderMode.Normal)),
+    # every MicrofacetBSDF constructor (microfacet.rs), glass in glass
+    "microfacet_spheres": lambda: (W.microfacet_spheres(), 64, 48, 4, RenderSettings(total_samples=4, max_bounces=10)),
 }
 
 

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvoidray_hip.so")
 
 VR_OK = 0
-VR_MAT_LAMBERTIAN, VR_MAT_METAL, VR_MAT_DIELECTRIC, VR_MAT_EMISSION, VR_MAT_LAMBERTIAN_BSDF = range(5)
+VR_MAT_LAMBERTIAN, VR_MAT_METAL, VR_MAT_DIELECTRIC, VR_MAT_EMISSION, VR_MAT_LAMBERTIAN_BSDF, VR_MAT_MICROFACET = range(6)
 VR_TEX_RGB32F, VR_TEX_RGBA8 = 0, 1
 VR_SAMPLE_NEAREST, VR_SAMPLE_BILINEAR = 0, 1
 VR_ENV_NONE, VR_ENV_UNIFORM, VR_ENV_HDRI = 0, 1, 2
@@ -42,7 +42,9 @@ class vr_mesh(C.Structure):
 
 class vr_material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("color", F3), ("param", C.c_float),
-                ("albedo_texture", C.c_int32), ("normal_texture", C.c_int32)]
+                ("albedo_texture", C.c_int32), ("normal_texture", C.c_int32),
+                ("index", C.c_float), ("roughness", C.c_float), ("metallic", C.c_float), ("emittance", C.c_float),
+                ("transparent", C.c_int32)]
 
 
 class vr_texture(C.Structure):

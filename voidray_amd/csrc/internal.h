@@ -91,6 +91,8 @@ struct GpuMaterial {
     float param;
     int32_t albedo_tex;
     int32_t normal_tex;
+    int32_t transparent;        // MicrofacetBSDF
+    float index, roughness, metallic;
     int32_t pad;
 };
 

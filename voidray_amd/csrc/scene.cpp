@@ -62,7 +62,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
         if (d->objects[i].material >= d->num_materials) return fail(VR_ERR_INVALID, "object material out of range");
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const vr_material& m = d->materials[i];
-        if (m.kind < 0 || m.kind > VR_MAT_LAMBERTIAN_BSDF) return fail(VR_ERR_INVALID, "unknown material kind");
+        if (m.kind < 0 || m.kind > VR_MAT_MICROFACET) return fail(VR_ERR_INVALID, "unknown material kind");
         if (m.albedo_texture >= (int32_t)d->num_textures || m.normal_texture >= (int32_t)d->num_textures)
             return fail(VR_ERR_INVALID, "material texture index out of range");
     }
@@ -290,7 +290,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         const vr_material& m = d->materials[i];
         mats[i] = GpuMaterial{m.kind, {m.color[0], m.color[1], m.color[2]}, m.param, m.albedo_texture,
-                              m.normal_texture, 0};
+                              m.normal_texture, m.transparent ? 1 : 0, m.index, m.roughness, m.metallic, 0};
     }
     std::vector<GpuTexture>& texs = H.textures;
     texs.assign(d->num_textures, GpuTexture{});

@@ -114,6 +114,159 @@ VR_HD vf3 refract3(vf3 uv, vf3 n, float eta) {
     vf3 par = (-sqrtf(fabsf(1.0f - mag2(perp)))) * n;
     return perp + par;
 }
+// ---- MicrofacetBSDF (voidray_common/src/microfacet.rs) ----------------------
+// powi as LLVM expands it (binary exponentiation): x^3 = x * x^2, x^5 = x * (x^2)^2.
+VR_HD float powi2(float x) { return x * x; }
+VR_HD float powi3(float x) { return x * (x * x); }
+VR_HD float powi5(float x) {
+    const float x2 = x * x;
+    return x * (x2 * x2);
+}
+// f32::is_normal: neither zero, subnormal, infinite nor NaN.
+VR_HD bool is_normal_f(float x) { return fabsf(x) >= 1.17549435e-38f && fabsf(x) <= 3.40282347e38f; }
+// f32::is_sign_positive
+VR_HD bool sign_positive(float x) { return !(f2u(x) >> 31); }
+
+// local_to_world(n) * h (util/math.rs:50-59).  The reference fills the
+// matrix column by column with (ns.x, nss.x, n.x), (ns.y, nss.y, n.y), ...,
+// so the product is (ns·h, nss·h, n·h); reproduced as written.
+VR_HD vf3 local_to_world_mul(vf3 n, vf3 h) {
+    const vf3 ns = is_normal_f(n.x) ? normalize3(mk3(n.y, -n.x, 0.0f)) : normalize3(mk3(0.0f, -n.z, n.y));
+    const vf3 nss = cross3(n, ns);
+    return mk3(dot3(ns, h), dot3(nss, h), dot3(n, h));
+}
+
+// UnitCircle (rand_distr 0.4.3): von Neumann's rejection method.
+VR_HD vf2 unit_circle(PathRng& rng, uint32_t& capped) {
+    for (int i = 0; i < kRejectCap; ++i) {
+        const float x1 = rng.uniform() * 2.0f - 1.0f;
+        const float x2 = rng.uniform() * 2.0f - 1.0f;
+        const float sum = x1 * x1 + x2 * x2;
+        if (sum >= 1.0f) continue;
+        const float diff = x1 * x1 - x2 * x2;
+        return vf2{diff / sum, (2.0f * x1) * x2 / sum};
+    }
+    ++capped;
+    return vf2{1.f, 0.f};
+}
+
+// f32 exp, ln, sin, cos, atan as the host's libm gives them (Rust's f32::exp
+// etc. call glibc's expf, logf, ...): evaluated in f64 and rounded once, which
+// agrees with glibc to within one ulp.  The device's own f32 expf returns 0
+// for results in the denormal range (x < about -87.3), where glibc returns the
+// denormal; the Beckmann pdf reaches that range, and 0 there turns a sample
+// the reference keeps (tiny pdf, clamped firefly) into "no scatter".
+VR_HD float exp_libm(float x) { return (float)exp((double)x); }
+VR_HD float ln_libm(float x) { return (float)log((double)x); }
+VR_HD float sin_libm(float x) { return (float)sin((double)x); }
+VR_HD float cos_libm(float x) { return (float)cos((double)x); }
+VR_HD float atan_libm(float x) { return (float)atan((double)x); }
+
+// Beckmann distribution term D (microfacet.rs:140-143, 183-186).
+VR_HD float beckmann_d(float m2, float nh2) { return exp_libm((nh2 - 1.0f) / (m2 * nh2)) / (((m2 * kPi) * nh2) * nh2); }
+
+// MicrofacetBSDF::bsdf (microfacet.rs:120-213); n, wo, wi as the caller passes them.
+VR_HD vf3 microfacet_bsdf(const GpuMaterial& m, vf3 n, vf3 wo, vf3 wi) {
+    const vf3 color = mk3(m.color[0], m.color[1], m.color[2]);
+    const float n_dot_wi = dot3(n, wi), n_dot_wo = dot3(n, wo);
+    const bool wi_outside = sign_positive(n_dot_wi), wo_outside = sign_positive(n_dot_wo);
+    if (!m.transparent && (!wi_outside || !wo_outside)) return mk3(0.f, 0.f, 0.f);
+    const float m2 = m.roughness * m.roughness;
+    const float f0s = powi2((m.index - 1.0f) / (m.index + 1.0f));
+    const vf3 f0 = mk3(f0s, f0s, f0s) * (1.0f - m.metallic) + color * m.metallic;  // lerp (math.rs:40-45)
+    if (wi_outside == wo_outside) {
+        const vf3 h = normalize3(wi + wo);
+        const float wo_dot_h = dot3(wo, h), n_dot_h = dot3(n, h);
+        const float nh2 = powi2(n_dot_h);
+        const float d = beckmann_d(m2, nh2);
+        vf3 f;
+        if (!wi_outside && sqrtf(1.0f - wo_dot_h * wo_dot_h) * m.index > 1.0f) {
+            f = mk3(1.f, 1.f, 1.f);  // total internal reflection
+        } else {
+            const float p5 = powi5(1.0f - wo_dot_h);
+            f = f0 + (mk3(1.f, 1.f, 1.f) - f0) * p5;
+        }
+        float g = fminf(n_dot_wi * n_dot_h, n_dot_wo * n_dot_h);
+        g = (2.0f * g) / wo_dot_h;
+        g = fminf(g, 1.0f);
+        const vf3 specular = ((d * f) * g) / ((4.0f * n_dot_wo) * n_dot_wi);
+        if (m.transparent) return specular;
+        const vf3 diffuse = mulv(mk3(1.f, 1.f, 1.f) - f, color) / kPi;
+        return specular + diffuse;
+    }
+    const float eta_t = wo_outside ? m.index : 1.0f / m.index;
+    const vf3 h = normalize3(wi * eta_t + wo);
+    const float wi_dot_h = dot3(wi, h), wo_dot_h = dot3(wo, h), n_dot_h = dot3(n, h);
+    const float nh2 = powi2(n_dot_h);
+    const float d = beckmann_d(m2, nh2);
+    const float p5 = powi5(1.0f - fabsf(wi_dot_h));
+    const vf3 f = f0 + (mk3(1.f, 1.f, 1.f) - f0) * p5;
+    float g = fminf(fabsf(n_dot_wi * n_dot_h), fabsf(n_dot_wo * n_dot_h));
+    g = (2.0f * g) / fabsf(wo_dot_h);
+    g = fminf(g, 1.0f);
+    const float a = fabsf((wi_dot_h * wo_dot_h) / (n_dot_wi * n_dot_wo));
+    const vf3 btdf = a * (((d * (mk3(1.f, 1.f, 1.f) - f)) * g) / powi2(eta_t * wi_dot_h + wo_dot_h));
+    return mulv(btdf, color);
+}
+
+// Beckmann pdf of a half vector (microfacet.rs:245-250).
+VR_HD float beckmann_pdf(float m2, vf3 n, vf3 h) {
+    const float cos_t = fabsf(dot3(h, n));
+    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    return (1.0f / ((kPi * m2) * powi3(cos_t))) * exp_libm((-powi2(sin_t / cos_t)) / m2);
+}
+
+// MicrofacetBSDF::sample (microfacet.rs:215-312); false = None.
+VR_HD bool microfacet_sample(const GpuMaterial& m, vf3 n, vf3 wo, PathRng& rng, uint32_t& capped, vf3& wi,
+                             float& pdf) {
+    const vf3 color = mk3(m.color[0], m.color[1], m.color[2]);
+    const float m2 = m.roughness * m.roughness;
+    const float f0 = powi2((m.index - 1.0f) / (m.index + 1.0f));
+    float f = (1.0f - m.metallic) * f0 + m.metallic * (((color.x + color.y) + color.z) / 3.0f);  // Color::mean
+    f = f * (1.0f - 0.2f) + 1.0f * 0.2f;                                                       // lerp(f, 1, 0.2)
+    const float eta_t = dot3(wo, n) > 0.0f ? m.index : 1.0f / m.index;
+    auto beckmann = [&]() {
+        // θ = arctan √(-m^2 ln U), azimuth from UnitCircle
+        const float theta = atan_libm(sqrtf(m2 * -ln_libm(rng.uniform())));
+        const float sin_t = sin_libm(theta), cos_t = cos_libm(theta);
+        const vf2 c = unit_circle(rng, capped);
+        return local_to_world_mul(n, mk3(c.x * sin_t, c.y * sin_t, cos_t));
+    };
+    if (rng.uniform() < f) {  // gen_bool(f)
+        wi = -reflect3(wo, beckmann());
+    } else if (!m.transparent) {
+        const vf2 q = unit_disc(rng, capped);  // Malley's method
+        const float z = sqrtf((1.0f - q.x * q.x) - q.y * q.y);
+        wi = local_to_world_mul(n, mk3(q.x, q.y, z));
+    } else {
+        const vf3 h = beckmann();
+        const float cos_to = dot3(h, wo);
+        const vf3 wo_perp = wo - h * cos_to;
+        const vf3 wi_perp = (-wo_perp) / eta_t;
+        const float sin2_ti = mag2(wi_perp);
+        if (sin2_ti > 1.0f) return false;  // total internal reflection
+        const float cos_ti = sqrtf(1.0f - sin2_ti);
+        wi = ((-copysignf(1.0f, cos_to)) * cos_ti) * h + wi_perp;
+    }
+    float p = 0.0f;
+    {
+        const vf3 h = normalize3(wi + wo);
+        p += f * beckmann_pdf(m2, n, h) / (4.0f * fabsf(dot3(h, wo)));
+    }
+    if (!m.transparent) {
+        p += (1.0f - f) * fmaxf(dot3(wi, n), 0.0f) / kPi;
+    } else if (sign_positive(dot3(wo, n)) != sign_positive(dot3(wi, n))) {
+        const vf3 h = normalize3(wi * eta_t + wo);
+        const float p_h = beckmann_pdf(m2, n, h);
+        const float h_dot_wo = dot3(h, wo), h_dot_wi = dot3(h, wi);
+        const float jacobian = fabsf(h_dot_wo) / powi2(eta_t * h_dot_wi + h_dot_wo);
+        p += ((1.0f - f) * p_h) * jacobian;
+    }
+    if (p == 0.0f) return false;
+    pdf = p;
+    return true;
+}
+
 // Dielectric::reflectance (simple.rs:192-197); powi(x, 5) = x * (x*x)^2
 VR_HD float schlick(float c, float idx) {
     float r0 = (1.0f - idx) / (1.0f + idx);
@@ -236,6 +389,23 @@ VR_HD ScatterOut scatter(const SceneView& sv, const GpuMaterial& m, vf3 d, const
             s.att = mk3(m.color[0], m.color[1], m.color[2]);
             s.has_ray = false;
             return s;
+        case VR_MAT_MICROFACET: {
+            // BSDFMaterial::scatter (traits.rs:19-41): wo is the ray's own
+            // direction, normalised, as the reference passes it
+            const vf3 wo = normalize3(d);
+            vf3 wi;
+            float pdf;
+            if (!microfacet_sample(m, hit.normal, wo, rng, capped, wi, pdf)) {
+                s.att = mk3(0.f, 0.f, 0.f);
+                s.has_ray = false;
+                return s;
+            }
+            const vf3 f = microfacet_bsdf(m, hit.normal, wo, wi);
+            s.dir = normalize3(wi);
+            s.att = (f * fabsf(dot3(wi, hit.normal))) * (1.0f / pdf);
+            s.has_ray = true;
+            return s;
+        }
         case VR_MAT_LAMBERTIAN_BSDF: {
             if (mag2(hit.normal) == 0.0f) {  // the reference asserts here
                 ++capped;

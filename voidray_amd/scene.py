@@ -50,6 +50,44 @@ class Material:
     param: float = 0.0
     albedo_texture: int = -1
     normal_texture: int = -1
+    index: float = 1.5          # MicrofacetBSDF parameters (microfacet.rs:9-27)
+    roughness: float = 1.0
+    metallic: float = 0.0
+    emittance: float = 0.0
+    transparent: bool = False
+
+
+class MicrofacetBSDF:
+    """Constructors of voidray_common/src/microfacet.rs:29-102 (f32 parameters)."""
+
+    @staticmethod
+    def _make(color, index, roughness, metallic, emittance, transparent):
+        return Material(N.VR_MAT_MICROFACET, tuple(_f3(color)), 0.0, -1, -1, float(f32(index)),
+                        float(f32(roughness)), float(f32(metallic)), float(f32(emittance)), bool(transparent))
+
+    @staticmethod
+    def diffuse(color):
+        return MicrofacetBSDF._make(color, 1.5, 1.0, 0.0, 0.0, False)
+
+    @staticmethod
+    def specular(color, roughness):
+        return MicrofacetBSDF._make(color, 1.5, roughness, 0.0, 0.0, False)
+
+    @staticmethod
+    def clear(index, roughness):
+        return MicrofacetBSDF._make((1.0, 1.0, 1.0), index, roughness, 0.0, 0.0, True)
+
+    @staticmethod
+    def transparent(color, index, roughness):
+        return MicrofacetBSDF._make(color, index, roughness, 0.0, 0.0, True)
+
+    @staticmethod
+    def metallic(color, roughness):
+        return MicrofacetBSDF._make(color, 1.5, roughness, 1.0, 0.0, False)
+
+    @staticmethod
+    def light(color, emittance):
+        return MicrofacetBSDF._make(color, 1.0, 1.0, 0.0, emittance, False)
 
 
 class Materials:
@@ -314,7 +352,8 @@ class Scene:
             objs[i] = N.vr_object(s, m)
         mats = (N.vr_material * max(1, len(self.materials)))()
         for i, m in enumerate(self.materials):
-            mats[i] = N.vr_material(m.kind, N.F3(*m.color), m.param, m.albedo_texture, m.normal_texture)
+            mats[i] = N.vr_material(m.kind, N.F3(*m.color), m.param, m.albedo_texture, m.normal_texture,
+                                    m.index, m.roughness, m.metallic, m.emittance, int(m.transparent))
         texs = (N.vr_texture * max(1, len(self.textures)))()
         for i, t in enumerate(self.textures):
             texs[i] = N.vr_texture(t.data.shape[1], t.data.shape[0], t.format, t.sample_type, ptr(t.data))

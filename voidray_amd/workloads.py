@@ -466,3 +466,32 @@ def analytic_and_meshes(seed=13):
     scene.add_object(mat, scene.add_mesh(random_triangles(2000, seed)))
     scene.add_object(mat, scene.add_analytic_surface(Surfaces.sphere((0.0, 2.0, 0.0), 0.5)))
     return scene
+
+
+def microfacet_spheres(sky=(64, 32)):
+    """Every MicrofacetBSDF constructor (voidray_common/src/microfacet.rs:29-102)
+    on analytic spheres: clear and tinted glass at several roughnesses, rough
+    specular and metallic spheres, a diffuse one and a `light` one (whose
+    emittance the reference never reads), over a Lambertian ground plane,
+    lit by an HDRI.  The reference passes the ray's own direction as wo, so
+    the opaque constructors shade black from outside; the glass ones
+    transmit and reflect."""
+    from .scene import MicrofacetBSDF, Surfaces
+    scene = Scene.empty()
+    mats = [MicrofacetBSDF.clear(1.5, 0.05), MicrofacetBSDF.clear(1.33, 0.4),
+            MicrofacetBSDF.transparent(_hex(0x7CA3E7), 1.5, 0.2), MicrofacetBSDF.transparent(_hex(0xE78999), 2.0, 0.8),
+            MicrofacetBSDF.specular(_hex(0xB3E7AA), 0.3), MicrofacetBSDF.metallic(_hex(0xE7C27C), 0.15),
+            MicrofacetBSDF.diffuse(_hex(0xAAAAAA)), MicrofacetBSDF.light((1.0, 0.9, 0.8), 5.0)]
+    for i, m in enumerate(mats):
+        x = -2.1 + 0.6 * i
+        scene.add_object(scene.add_material(m), scene.add_analytic_surface(Surfaces.sphere((x, 0.3, 0.0), 0.28)))
+    # a glass shell around a coloured glass core
+    scene.add_object(scene.add_material(MicrofacetBSDF.clear(1.5, 0.1)),
+                     scene.add_analytic_surface(Surfaces.sphere((0.0, 1.0, 0.6), 0.5)))
+    scene.add_object(scene.add_material(MicrofacetBSDF.transparent(_hex(0xB3E7AA), 1.2, 0.3)),
+                     scene.add_analytic_surface(Surfaces.sphere((0.0, 1.0, 0.6), 0.3)))
+    scene.add_object(scene.add_material(Materials.lambertian((0.6, 0.55, 0.5))),
+                     scene.add_analytic_surface(Surfaces.ground_plane(0.0)))
+    scene.camera = Camera.look_at((0.0, 1.2, -3.5), (0.0, 0.5, 0.0), (0.0, 1.0, 0.0), 0.9)
+    scene.environment = Environments.hdri(procedural_sky(*sky, sun_peak=50.0, sun_sharpness=40.0))
+    return scene
