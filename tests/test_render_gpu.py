@@ -50,6 +50,9 @@ CASES = {
                                      RenderSettings(total_samples=2, render_mode=RenderMode.Normal)),
     # every MicrofacetBSDF constructor (microfacet.rs), glass in glass
     "microfacet_spheres": lambda: (W.microfacet_spheres(), 64, 48, 4, RenderSettings(total_samples=4, max_bounces=10)),
+    # the reference's cornell example as written (555 scale, analytic glass sphere) and its material example
+    "cornell_demo": lambda: (W.cornell_demo(), 48, 48, 4, RenderSettings(total_samples=4, max_bounces=8)),
+    "material_demo": lambda: (W.material_demo(), 48, 48, 4, RenderSettings(total_samples=4, max_bounces=10)),
 }
 
 

@@ -112,3 +112,16 @@ def test_analytic_spheres_demo(oracle_lib):
     hits = wide["prim"] != 0xFFFFFFFF
     assert hits.mean() > 0.5
     assert wide["stats"]["num_triangles"] == 0
+
+
+def test_cornell_demo_scale(oracle_lib):
+    # the reference's cornell example at its own 555 scale: zero-normal quads
+    # and two concentric analytic spheres 0.1 apart
+    scene = W.cornell_demo()
+    rng = np.random.default_rng(10)
+    o = rng.uniform(5.0, 550.0, (4096, 3)).astype(np.float32)
+    o[:1024] = np.float32([278.0, 278.0, -800.0])
+    d = W._normalize_rows(rng.normal(size=(4096, 3)).astype(np.float32))
+    d[:1024] = W._normalize_rows((np.float32([277.5, 100.0, 277.5]) + rng.normal(scale=60.0, size=(1024, 3)) - o[:1024]).astype(np.float32))
+    wide = _check_all(scene, o, d, oracle_lib, "cornell-demo")
+    assert (wide["prim"] != 0xFFFFFFFF).mean() > 0.8  # the room is open at the front

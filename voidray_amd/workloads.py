@@ -495,3 +495,63 @@ def microfacet_spheres(sky=(64, 32)):
     scene.camera = Camera.look_at((0.0, 1.2, -3.5), (0.0, 0.5, 0.0), (0.0, 1.0, 0.0), 0.9)
     scene.environment = Environments.hdri(procedural_sky(*sky, sun_peak=50.0, sun_sharpness=40.0))
     return scene
+
+
+def cornell_demo():
+    """The reference's cornell example (voidray_app/src/examples/cornell.rs:5-78)
+    as written: Surfaces::quad walls at the 555 scale (vertex normals zero), its
+    albedos, an emissive quad of 15, and an analytic glass sphere (IOR 1.33)
+    around a blue Lambertian core 0.1 smaller."""
+    from .scene import Surfaces
+    scene = Scene.empty()
+    red = scene.add_material(Materials.lambertian((0.65, 0.05, 0.05)))
+    white = scene.add_material(Materials.lambertian((0.73, 0.73, 0.73)))
+    green = scene.add_material(Materials.lambertian((0.12, 0.45, 0.15)))
+    light = scene.add_material(Materials.emissive(15.0))
+    q = lambda *pts: scene.add_mesh(Surfaces.quad(*[np.asarray(p, np.float64) for p in pts]))
+    floor = q((0, 0, 0), (0, 0, 555), (555, 0, 555), (555, 0, 0))
+    red_wall = q((0, 0, 0), (0, 0, 555), (0, 555, 555), (0, 555, 0))
+    green_wall = q((555, 0, 0), (555, 0, 555), (555, 555, 555), (555, 555, 0))
+    back_wall = q((0, 0, 555), (555, 0, 555), (555, 555, 555), (0, 555, 555))
+    ceil = q((0, 555, 0), (0, 555, 555), (555, 555, 555), (555, 555, 0))
+    light_plane = q((213, 554, 227), (213, 554, 332), (343, 554, 332), (343, 554, 227))
+    for mat, surf in ((white, floor), (green, green_wall), (red, red_wall), (white, back_wall), (white, ceil),
+                      (light, light_plane)):
+        scene.add_object(mat, surf)
+    sph = scene.add_analytic_surface(Surfaces.sphere((555.0 / 2.0, 100.0, 555.0 / 2.0), 100.0))
+    glass = scene.add_material(Materials.dielectric(1.33))
+    sph_inner = scene.add_analytic_surface(Surfaces.sphere((555.0 / 2.0, 100.0, 555.0 / 2.0), 99.9))
+    glass_inner = scene.add_material(Materials.lambertian(_hex(0x0F1BF0)))
+    scene.add_object(glass, sph)
+    scene.add_object(glass_inner, sph_inner)
+    scene.camera = Camera.look_at((278.0, 278.0, -800.0), (278.0, 278.0, 0.0), (0.0, 1.0, 0.0),
+                                  degrees_to_radians(40.0))
+    return scene
+
+
+def uv_test_texture(size=64):
+    """An 8x8 checker of distinct colours, standing in for assets/uv_test.png."""
+    y, x = np.mgrid[0:size, 0:size]
+    cx, cy = (x * 8) // size, (y * 8) // size
+    img = np.zeros((size, size, 3), np.float32)
+    img[..., 0] = (cx + 1) / 8.0
+    img[..., 1] = (cy + 1) / 8.0
+    img[..., 2] = np.where((cx + cy) % 2 == 0, 0.9, 0.2)
+    return img
+
+
+def material_demo(tex_size=64, sky=(64, 32)):
+    """The reference's material example (voidray_app/src/examples/material.rs:3-30):
+    a LambertianBSDF sphere on an analytic ground plane with a nearest-sampled
+    texture and no normal map, the default camera moved to (0.5, 0.6, 5), an
+    HDRI; the image assets are replaced by a procedural checker and sky."""
+    from .scene import Surfaces
+    scene = Scene.empty()
+    ground = scene.add_analytic_surface(Surfaces.ground_plane(0.0))
+    uv = scene.add_image_texture(uv_test_texture(tex_size), SampleType.Nearest)
+    scene.add_object(scene.add_material(Materials.lambertian_texture_no_normal(uv)), ground)
+    sphere = scene.add_analytic_surface(Surfaces.sphere((0.0, 0.5, 0.0), 0.5))
+    scene.add_object(scene.add_material(Materials.lambertian_bsdf(_hex(0xFF3030))), sphere)
+    scene.camera.eye = np.asarray((0.5, 0.6, 5.0), np.float32)
+    scene.environment = Environments.hdri(procedural_sky(*sky, sun_peak=50.0, sun_sharpness=40.0))
+    return scene
