@@ -43,6 +43,16 @@ typedef struct vr_mesh {
     const uint32_t* indices;/* 3 * num_triangles */
 } vr_mesh;
 
+/* Mesh::from_file (mesh.rs:46-74): reads a Wavefront OBJ as obj-rs 0.7 does
+ * for Obj<TexturedVertex, u32> (triangles with position/texture/normal
+ * references; vertices = distinct reference triples in order of first use).
+ * On success *out holds library-allocated arrays (normals and uvs set), to be
+ * given back with vr_mesh_release.  Errors (unreadable file, untriangulated
+ * faces, missing uv or normal references, indices out of range) return
+ * VR_ERR_INVALID where the reference panics.  Host only; no GPU needed. */
+int vr_mesh_load_obj(const char* path, vr_mesh* out);
+int vr_mesh_release(vr_mesh* mesh);
+
 /* Materials of voidray_common/src/simple.rs:17-58. */
 #define VR_MAT_LAMBERTIAN 0      /* Lambertian::scatter  simple.rs:102-132 */
 #define VR_MAT_METAL 1           /* Metal::scatter       simple.rs:140-160 */
@@ -418,6 +428,21 @@ int vr_cache_last_stats(vr_cache* cache, vr_cache_stats* stats);
 #define VR_TONEMAP_FILMIC 3
 #define VR_TONEMAP_UNCHARTED2 4
 
+/* The legacy viewer's pass (voidray/src/shaders/viewport_fragment.glsl with
+ * voidray/src/shaders/tonemapping.glsl; ids are Tonemap::to_int,
+ * voidray/src/pipeline.rs:86-97, plus 16).  It differs from the pass above:
+ * exposure is a linear factor, not a power of two; Filmic forces gamma = 1;
+ * alpha is scale * source alpha when keep_alpha is set (the viewer's
+ * `transparent`), else 1. */
+#define VR_TONEMAP_LEGACY_NONE 16
+#define VR_TONEMAP_LEGACY_SIMPLE_ACES 17
+#define VR_TONEMAP_LEGACY_SIMPLE_REINHARD 18
+#define VR_TONEMAP_LEGACY_LUMA_REINHARD 19
+#define VR_TONEMAP_LEGACY_LUMA_WHITE_PRESERVING_REINHARD 20
+#define VR_TONEMAP_LEGACY_ROM_BIN_DA_HOUSE 21
+#define VR_TONEMAP_LEGACY_FILMIC 22
+#define VR_TONEMAP_LEGACY_UNCHARTED2 23
+
 typedef struct vr_post_desc {
     uint32_t width, height;
     float scale;     /* total/current samples (voidray_app/src/main.rs:75-79) */
@@ -425,6 +450,7 @@ typedef struct vr_post_desc {
     float exposure;
     int32_t tonemap; /* VR_TONEMAP_* (Tonemap::as_i32, settings.rs:44-54) */
     uint32_t clip_to_1024; /* reproduce the fixed 1024x1024 dispatch (post_process.rs:74) */
+    uint32_t keep_alpha;   /* legacy modes only: output alpha = scale * source alpha */
 } vr_post_desc;
 
 int vr_tonemap(vr_ctx* ctx, const vr_post_desc* desc, const float* src_rgba, float* dst_rgba);

@@ -1386,13 +1386,54 @@ static void uncharted2(float c[3]) {
     }
 }
 
-// FP64 evaluation of the same formulas, for pinning the f32 versions.
+// The legacy viewer's curves (voidray/src/shaders/tonemapping.glsl:17-45).
+static void simple_reinhard(float c[3]) {
+    const float exposure = 1.5f;
+    for (int i = 0; i < 3; ++i) c[i] *= exposure / (1.f + c[i] / exposure);
+}
+static void luma_reinhard(float c[3]) {
+    float luma = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+    float tl = luma / (1.f + luma);
+    for (int i = 0; i < 3; ++i) c[i] *= tl / luma;
+}
+static void rombindahouse(float c[3]) {
+    for (int i = 0; i < 3; ++i) c[i] = std::exp(-1.0f / (2.72f * c[i] + 0.15f));
+}
+
+// voidray/src/shaders/viewport_fragment.glsl:27-67.
+static void legacy_post(const vr_post_desc* p, const float* s, float* d) {
+    float c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (s[k] * p->scale) * p->exposure;
+    float gamma = p->gamma;
+    switch (p->tonemap) {
+        case VR_TONEMAP_LEGACY_SIMPLE_ACES: aces(c); break;
+        case VR_TONEMAP_LEGACY_SIMPLE_REINHARD: simple_reinhard(c); break;
+        case VR_TONEMAP_LEGACY_LUMA_REINHARD: luma_reinhard(c); break;
+        case VR_TONEMAP_LEGACY_LUMA_WHITE_PRESERVING_REINHARD: reinhard(c); break;
+        case VR_TONEMAP_LEGACY_ROM_BIN_DA_HOUSE: rombindahouse(c); break;
+        case VR_TONEMAP_LEGACY_FILMIC:
+            filmic(c);
+            gamma = 1.f;
+            break;
+        case VR_TONEMAP_LEGACY_UNCHARTED2: uncharted2(c); break;
+        default: break;
+    }
+    for (int k = 0; k < 3; ++k) d[k] = std::pow(c[k], 1.f / gamma);
+    d[3] = p->keep_alpha ? s[3] * p->scale : 1.f;
+}
+
+// f32 evaluation of the published formulas (the tests pin it against FP64
+// closed forms).
 void orc_tonemap(const vr_post_desc* p, const float* src, float* dst) {
     uint32_t W = p->width, H = p->height;
     for (uint32_t y = 0; y < H; ++y)
         for (uint32_t x = 0; x < W; ++x) {
             size_t i = (size_t)y * W + x;
             if (p->clip_to_1024 && (x >= 1024 || y >= 1024)) continue;
+            if (p->tonemap >= VR_TONEMAP_LEGACY_NONE) {
+                legacy_post(p, src + 4 * i, dst + 4 * i);
+                continue;
+            }
             float c[3];
             float ex = std::pow(2.f, p->exposure);
             for (int k = 0; k < 3; ++k) c[k] = (src[4 * i + k] * p->scale) * ex;

@@ -339,6 +339,43 @@ def test_tonemap_curves_fp64():
     assert np.allclose(run(0, gamma=2.2, exposure=1.0, scale=0.5)[..., :3], (x * 0.5 * 2.0) ** (1 / 2.2), rtol=1e-5)
 
 
+def test_legacy_viewer_tonemaps_fp64():
+    # voidray/src/shaders/viewport_fragment.glsl + tonemapping.glsl, FP64 closed forms
+    c = np.array([[[0.2, 0.5, 1.5, 0.8], [2.0, 0.01, 0.7, 0.25]]], f32)
+    def run(mode, gamma=1.0, exposure=1.0, scale=1.0, keep_alpha=0):
+        return oracle.tonemap(c, N.vr_post_desc(2, 1, scale, gamma, exposure, mode, 0, keep_alpha))
+    x = c[..., :3].astype(np.float64)
+    a = c[..., 3].astype(np.float64)
+    # linear exposure and scale, gamma, alpha 1 unless kept (then scale * alpha)
+    out = run(N.VR_TONEMAP_LEGACY_NONE, gamma=2.2, exposure=3.0, scale=0.5)
+    assert np.allclose(out[..., :3], (x * 1.5) ** (1 / 2.2), rtol=1e-6) and np.all(out[..., 3] == 1)
+    out = run(N.VR_TONEMAP_LEGACY_NONE, scale=0.5, keep_alpha=1)
+    assert np.allclose(out[..., 3], a * 0.5, rtol=1e-7)
+    # simple Reinhard with its own exposure 1.5
+    assert np.allclose(run(N.VR_TONEMAP_LEGACY_SIMPLE_REINHARD)[..., :3], x * 1.5 / (1 + x / 1.5), rtol=1e-6)
+    # luma Reinhard
+    l = x @ np.array([0.2126, 0.7152, 0.0722])
+    assert np.allclose(run(N.VR_TONEMAP_LEGACY_LUMA_REINHARD)[..., :3], x * ((l / (1 + l)) / l)[..., None], rtol=1e-6)
+    # white-preserving luma Reinhard, white = 2
+    tl = l * (1 + l / 4) / (1 + l)
+    assert np.allclose(run(N.VR_TONEMAP_LEGACY_LUMA_WHITE_PRESERVING_REINHARD)[..., :3], x * (tl / l)[..., None],
+                       rtol=1e-6)
+    # RomBinDaHouse
+    assert np.allclose(run(N.VR_TONEMAP_LEGACY_ROM_BIN_DA_HOUSE)[..., :3], np.exp(-1 / (2.72 * x + 0.15)), rtol=1e-6)
+    # Filmic ignores the gamma setting
+    y = np.maximum(0, x - 0.004)
+    assert np.allclose(run(N.VR_TONEMAP_LEGACY_FILMIC, gamma=2.2)[..., :3],
+                       (y * (6.2 * y + .5)) / (y * (6.2 * y + 1.7) + 0.06), rtol=1e-6)
+    # ACES and Uncharted 2 are the curves of the current pass; exposure 1 here = 2^0 there
+    assert np.array_equal(run(N.VR_TONEMAP_LEGACY_SIMPLE_ACES)[..., :3], run_new(c, N.VR_TONEMAP_ACES)[..., :3])
+    assert np.array_equal(run(N.VR_TONEMAP_LEGACY_UNCHARTED2, gamma=2.2)[..., :3],
+                          run_new(c, N.VR_TONEMAP_UNCHARTED2, gamma=2.2)[..., :3])
+
+
+def run_new(c, mode, gamma=1.0):
+    return oracle.tonemap(c, N.vr_post_desc(c.shape[1], c.shape[0], 1.0, gamma, 0.0, mode, 0))
+
+
 def test_camera_look_at_is_orthonormal():
     cam = Camera.look_at((1, 2, 3), (0, 0, 0), (0, 1, 0), 0.7)
     assert abs(float(np.dot(cam.direction, cam.up))) < 1e-6

@@ -141,11 +141,13 @@ def test_cornell_full_size_rows(gpu_ctx, oracle_lib):
 
 
 @pytest.mark.parametrize("mode", [N.VR_TONEMAP_NONE, N.VR_TONEMAP_ACES, N.VR_TONEMAP_REINHARD, N.VR_TONEMAP_FILMIC,
-                                  N.VR_TONEMAP_UNCHARTED2])
+                                  N.VR_TONEMAP_UNCHARTED2] + list(range(N.VR_TONEMAP_LEGACY_NONE,
+                                                                        N.VR_TONEMAP_LEGACY_UNCHARTED2 + 1)))
 def test_tonemap_matches_oracle(mode, gpu_ctx, oracle_lib):
     rng = np.random.default_rng(mode)
     src = np.abs(rng.normal(0.5, 1.0, (37, 53, 4))).astype(np.float32) + 0.01
-    post = N.vr_post_desc(53, 37, 1.5, 2.2, 0.5, mode, 0)
+    # legacy modes: linear exposure, alpha kept on every other mode
+    post = N.vr_post_desc(53, 37, 1.5, 2.2, 0.5 if mode < N.VR_TONEMAP_LEGACY_NONE else 1.7, mode, 0, mode & 1)
     announce(f"tonemap mode {mode}")
     got = tonemap(gpu_ctx, src, post)
     ref = oracle_lib.tonemap(src, post)

@@ -30,6 +30,9 @@ VR_BATCH_GRAPH = 1
 VR_BATCH_TIMESTAMPS = 2
 VR_BATCH_SHORTEST_FIRST = 4
 VR_TONEMAP_NONE, VR_TONEMAP_ACES, VR_TONEMAP_REINHARD, VR_TONEMAP_FILMIC, VR_TONEMAP_UNCHARTED2 = range(5)
+(VR_TONEMAP_LEGACY_NONE, VR_TONEMAP_LEGACY_SIMPLE_ACES, VR_TONEMAP_LEGACY_SIMPLE_REINHARD,
+ VR_TONEMAP_LEGACY_LUMA_REINHARD, VR_TONEMAP_LEGACY_LUMA_WHITE_PRESERVING_REINHARD,
+ VR_TONEMAP_LEGACY_ROM_BIN_DA_HOUSE, VR_TONEMAP_LEGACY_FILMIC, VR_TONEMAP_LEGACY_UNCHARTED2) = range(16, 24)
 
 F3 = C.c_float * 3
 
@@ -106,7 +109,8 @@ class vr_render_stats(C.Structure):
 
 class vr_post_desc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("scale", C.c_float), ("gamma", C.c_float),
-                ("exposure", C.c_float), ("tonemap", C.c_int32), ("clip_to_1024", C.c_uint32)]
+                ("exposure", C.c_float), ("tonemap", C.c_int32), ("clip_to_1024", C.c_uint32),
+                ("keep_alpha", C.c_uint32)]
 
 
 class vr_request(C.Structure):
@@ -153,6 +157,8 @@ SIGNATURES = {
     "vr_destroy": (C.c_int, [_P]),
     "vr_last_error": (C.c_char_p, []),
     "vr_stream": (_P, [_P]),
+    "vr_mesh_load_obj": (C.c_int, [C.c_char_p, C.POINTER(vr_mesh)]),
+    "vr_mesh_release": (C.c_int, [C.POINTER(vr_mesh)]),
     "vr_scene_upload": (C.c_int, [_P, C.POINTER(vr_scene_desc), C.POINTER(_P)]),
     "vr_scene_release": (C.c_int, [_P]),
     "vr_scene_get_stats": (C.c_int, [_P, C.POINTER(vr_scene_stats)]),

@@ -17,6 +17,7 @@ Image and mesh data are given as arrays (the reference reads files through
 the image and obj crates; their decoders are not part of this path).
 """
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -150,6 +151,21 @@ class Mesh:
     def from_buffers(positions, indices, normals=None, uvs=None):
         return Mesh(positions, indices, normals, uvs)
 
+    @staticmethod
+    def from_file(path):
+        """Mesh::from_file (mesh.rs:46-74): a Wavefront OBJ read by the library's
+        loader (obj-rs 0.7 semantics for TexturedVertex meshes)."""
+        m = N.vr_mesh()
+        N.check(N.lib().vr_mesh_load_obj(os.fsencode(path), C.byref(m)))
+        try:
+            nv, nt = m.num_vertices, m.num_triangles
+            arr = lambda ptr, n, ct: np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), (n,)).copy() if n else \
+                np.zeros(0, np.float32 if ct is C.c_float else np.uint32)
+            return Mesh(arr(m.positions, 3 * nv, C.c_float), arr(m.indices, 3 * nt, C.c_uint32),
+                        arr(m.normals, 3 * nv, C.c_float), arr(m.uvs, 2 * nv, C.c_float))
+        finally:
+            N.lib().vr_mesh_release(C.byref(m))
+
     @property
     def num_triangles(self):
         return len(self.indices)
@@ -218,6 +234,11 @@ class Environments:
 
     @staticmethod
     def hdri(image):
+        """environments.rs:14-16: an (H, W, 3) array, or an image file read as
+        image::open(path).to_rgb32f() (OpenEXR, PNG, TIFF, ...)."""
+        if isinstance(image, (str, os.PathLike)):
+            from .imageio import load_image
+            image = load_image(image, want_float=True)
         return Environment(N.VR_ENV_HDRI, (0.0, 0.0, 0.0), np.ascontiguousarray(image, dtype=f32))
 
 
@@ -268,6 +289,18 @@ class Tonemap:
     Reinhard = N.VR_TONEMAP_REINHARD
     Filmic = N.VR_TONEMAP_FILMIC
     Uncharted2 = N.VR_TONEMAP_UNCHARTED2
+
+
+class LegacyTonemap:
+    """The legacy viewer's Tonemap (voidray/src/pipeline.rs:59-97), for vr_post_desc.tonemap."""
+    None_ = N.VR_TONEMAP_LEGACY_NONE
+    SimpleACES = N.VR_TONEMAP_LEGACY_SIMPLE_ACES
+    SimpleReinhard = N.VR_TONEMAP_LEGACY_SIMPLE_REINHARD
+    LumaReinhard = N.VR_TONEMAP_LEGACY_LUMA_REINHARD
+    LumaWhitePreservingReinhard = N.VR_TONEMAP_LEGACY_LUMA_WHITE_PRESERVING_REINHARD
+    RomBinDaHouse = N.VR_TONEMAP_LEGACY_ROM_BIN_DA_HOUSE
+    Filmic = N.VR_TONEMAP_LEGACY_FILMIC
+    Uncharted2 = N.VR_TONEMAP_LEGACY_UNCHARTED2
 
 
 @dataclass
@@ -325,8 +358,17 @@ class Scene:
         return len(self.objects) - 1
 
     def add_image_texture(self, data, sample_type=SampleType.Bilinear):
+        """scene.rs:154-160: an array ((H, W, 4) uint8 or (H, W, 3) float32) or an
+        image file path, read as image::open(path).to_rgb32f()."""
+        if isinstance(data, (str, os.PathLike)):
+            from .imageio import load_image
+            data = load_image(data)
         self.textures.append(Texture(np.asarray(data), sample_type))
         return len(self.textures) - 1
+
+    def add_mesh_from_file(self, path):
+        """scene.rs:137-144."""
+        return self.add_mesh(Mesh.from_file(path))
 
     @property
     def num_triangles(self):
