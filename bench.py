@@ -71,24 +71,30 @@ def _terrain(ctx):
     return _TERRAIN["scene"], _TERRAIN["accel"]
 
 
-def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8):
+def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=8,
+               warm_size=(240, 135, 2)):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
     + MIS) at width x height, judged against the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
-    Plain: progressive 4-spp passes, image snapshot after each.  Cached: 8
-    warm-up frames of 1 spp (rendered into a discarded image, counted in its
-    time), then 32 frames of 4 spp.  In each frame ~2^16 randomly chosen paths
-    run to the end as training paths and record up to 4 vertices with their
-    suffix radiance; the others stop at vertex `cache_depth` and add the
-    cache's answer; 8 Adam steps on disjoint slices of the samples follow.  The same
-    run with training off (untrained cache: truncation alone) is the control.
-    Device time per pass from HIP events; snapshots are excluded."""
+    Plain: progressive 4-spp passes, image snapshot after each.  Cached:
+    `warm_frames` warm-up frames that only train (the same camera at
+    warm_size = (w, h, spp), every path a training path, the image discarded;
+    counted in the cached run's time), then 32 frames of 4 spp.  In each frame
+    ~2^16 randomly chosen paths run to the end as training paths and record up
+    to 4 vertices with their suffix radiance; the others stop at vertex
+    `cache_depth` and add the cache's answer; 8 Adam steps on disjoint slices
+    of the samples follow.  The same run with training off (untrained cache:
+    truncation alone) is the control.  Every cache object first renders one
+    untimed frame without training, which sizes its buffers as the reference
+    render sized the plain tracer's.  Device time per pass from HIP events;
+    snapshots are excluded."""
     import numpy as np
 
     from voidray_amd import _native as N
     from voidray_amd.device import RadianceCache, make_render_desc, scene_bounds
     from voidray_amd.scene import RenderSettings
     sp = stream.cuda_stream
+    torch.cuda.set_stream(stream)  # snapshots (clone, cpu) must follow the renders on this stream
     scene, accel = _terrain(ctx)
     w, h = width, height
     ext = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
@@ -145,7 +151,8 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
            "reference_ms": round(ref_ms, 1),
            "plain": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5)} for t, s, e in plain_pts[::4] + plain_pts[-1:]]}
     lo, hi = scene_bounds(scene)
-    warm, frames = 8, 32
+    warm, frames = warm_frames, 32
+    ww, wh, wspp = warm_size
     scratch = torch.zeros_like(img)
 
     def box8(x):
@@ -156,16 +163,27 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     for depth in (2, 1):
         for trained in (True, False):
             cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=2048)
+            for sw, sh, sspp in ((w, h, spf), (ww, wh, wspp)):  # untimed buffer sizing, no training
+                cache.render_frame(accel, scratch.data_ptr(), make_render_desc(sw, sh, sspp, sspp, st, seed=3, flags=ext,
+                                                                               rr_depth=4),
+                                   train_paths=1 << 16, train_vertices=4, cache_depth=depth, train_steps=0, frame=0,
+                                   stream=sp)
+            torch.cuda.synchronize()
 
             def cached_pass(k, depth=depth, cache=cache, trained=trained):
-                # warm-up frames (1 spp) train the cache into a discarded image
-                target, off, n = (scratch, 0, 0) if k < warm else (img, (k - warm) * spf, spf)
-                cache.render_frame(accel, target.data_ptr(), make_render_desc(w, h, spf if n else 1, frames * spf, st,
-                                                                              sample_offset=off, seed=7, flags=ext,
-                                                                              rr_depth=4),
+                if k < warm:  # training-only frame: small image, every path trains, image discarded
+                    cache.render_frame(accel, scratch.data_ptr(),
+                                       make_render_desc(ww, wh, wspp, wspp, st, sample_offset=0, seed=11 + k, flags=ext,
+                                                        rr_depth=4),
+                                       train_paths=ww * wh * wspp, train_vertices=4, cache_depth=depth,
+                                       train_steps=train_steps if trained else 0, frame=k, stream=sp)
+                    return 0
+                cache.render_frame(accel, img.data_ptr(), make_render_desc(w, h, spf, frames * spf, st,
+                                                                           sample_offset=(k - warm) * spf, seed=7,
+                                                                           flags=ext, rr_depth=4),
                                    train_paths=1 << 16, train_vertices=4, cache_depth=depth,
                                    train_steps=train_steps if trained else 0, frame=k, stream=sp)
-                return n
+                return spf
 
             pts = curve(warm + frames, cached_pass, frames * spf)
             stats = cache.stats()
