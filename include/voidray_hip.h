@@ -157,6 +157,14 @@ typedef struct vr_scene_desc {
 
 #define VR_BUILD_DEFAULT 0
 #define VR_BUILD_NO_ENV_CDF 1u /* skip the environment importance-sampling tables */
+/* Leaf layout of the wide BVH.  Spatial splits put a triangle into every leaf
+ * holding a piece of it.  By default each leaf holds its 48-byte triangle
+ * records (the fastest layout).  VR_BUILD_LEAF_REFS makes leaves hold 4-byte
+ * references into one record per triangle instead: on the 1M-triangle
+ * random soup (5.7 records per triangle) that is a third less device memory
+ * for ~5% lower closest-hit throughput.  Same hits either way. */
+#define VR_BUILD_LEAF_RECORDS 2u /* the default, stated explicitly */
+#define VR_BUILD_LEAF_REFS 4u
 
 typedef struct vr_scene_stats {
     uint32_t num_triangles;
@@ -171,7 +179,7 @@ typedef struct vr_scene_stats {
     uint64_t device_bytes;
     uint32_t num_tri_records;   /* triangle records in the wide BVH's leaves: >= num_triangles,
                                    a triangle cut by spatial splits has one per leaf holding a piece */
-    uint32_t reserved;
+    uint32_t leaf_refs;         /* 1: leaves hold references into one record per triangle */
 } vr_scene_stats;
 
 /* ---- context ---------------------------------------------------------- */

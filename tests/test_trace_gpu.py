@@ -104,3 +104,21 @@ def test_full_size_microbench(gpu_ctx, oracle_lib):
     announce("trace full size: any-hit 4M rays")
     anyh = acc.hit(o, d, any_hit=True)["hit"]
     assert np.array_equal(anyh.astype(bool), wide["prim"] != 0xFFFFFFFF)
+
+
+def test_leaf_reference_layout(gpu_ctx, oracle_lib):
+    """Leaves holding 4-byte references into one record per triangle
+    (VR_BUILD_LEAF_REFS): wide == exact on every ray, a sample == the oracle."""
+    scene = W.random_triangle_scene(200000, seed=7)
+    o, d = W.microbench_rays(1 << 18, res=512, seed=8)
+    announce("trace leaf references: 200k tris, 262k rays")
+    acc = scene.build_acceleration(gpu_ctx, build_flags=N.VR_BUILD_LEAF_REFS)
+    assert acc.stats()["leaf_refs"] == 1
+    wide = acc.hit(o, d)
+    exact = acc.hit(o, d, engine=N.VR_ENGINE_REFERENCE)
+    _compare(exact, wide, "refs/wide-vs-exact")
+    sel = np.random.default_rng(1).choice(len(o), 2048, replace=False)
+    ref = oracle_lib.OracleScene(scene).trace(o[sel], d[sel])
+    _compare(ref, {k: v[sel] for k, v in wide.items()}, "refs/sample-vs-oracle")
+    anyh = acc.hit(o, d, any_hit=True)["hit"]
+    assert np.array_equal(anyh.astype(bool), wide["prim"] != 0xFFFFFFFF)

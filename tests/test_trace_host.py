@@ -16,13 +16,13 @@ from voidray_amd.device import trace_host
 from voidray_amd.scene import Scene
 
 
-def _check_all(scene, o, d, oracle_lib, label):
+def _check_all(scene, o, d, oracle_lib, label, build_flags=0):
     ref = oracle_lib.OracleScene(scene).trace(o, d)
-    wide = trace_host(scene, o, d)
+    wide = trace_host(scene, o, d, build_flags=build_flags)
     compare_hits(ref, wide, f"{label}/wide")
     exact = trace_host(scene, o, d, engine=N.VR_ENGINE_REFERENCE)
     compare_hits(ref, exact, f"{label}/exact")
-    anyh = trace_host(scene, o, d, any_hit=True)
+    anyh = trace_host(scene, o, d, any_hit=True, build_flags=build_flags)
     assert np.array_equal(anyh["hit"].astype(bool), ref["prim"] != 0xFFFFFFFF), f"{label}/any"
     return wide
 
@@ -37,6 +37,18 @@ def test_spatial_split_bvh_is_exact(oracle_lib):
     assert st["num_triangles"] == 20000
     assert 1.2 * st["num_triangles"] < st["num_tri_records"] <= 8 * st["num_triangles"], st
     assert wide["fallbacks"] < 8
+
+
+def test_leaf_layouts(oracle_lib):
+    # leaves holding the records themselves (the default) or 4-byte references
+    # into one record per triangle give the same answers
+    scene = W.random_triangle_scene(20000, seed=5)
+    o, d = W.microbench_rays(4096, res=64)
+    for flags, refs in ((N.VR_BUILD_LEAF_RECORDS, 0), (N.VR_BUILD_LEAF_REFS, 1)):
+        st = _check_all(scene, o, d, oracle_lib, f"soup20k/layout{flags}", build_flags=flags)["stats"]
+        assert st["leaf_refs"] == refs, st
+    assert trace_host(scene, o[:1], d[:1])["stats"]["leaf_refs"] == 0
+    _check_all(W.cornell_room(), o, d, oracle_lib, "cornell/refs", build_flags=N.VR_BUILD_LEAF_REFS)
 
 
 def test_multi_surface_scene(oracle_lib):

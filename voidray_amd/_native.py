@@ -15,7 +15,7 @@ VR_MAT_LAMBERTIAN, VR_MAT_METAL, VR_MAT_DIELECTRIC, VR_MAT_EMISSION, VR_MAT_LAMB
 VR_TEX_RGB32F, VR_TEX_RGBA8 = 0, 1
 VR_SAMPLE_NEAREST, VR_SAMPLE_BILINEAR = 0, 1
 VR_ENV_NONE, VR_ENV_UNIFORM, VR_ENV_HDRI = 0, 1, 2
-VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF = 0, 1
+VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF, VR_BUILD_LEAF_RECORDS, VR_BUILD_LEAF_REFS = 0, 1, 2, 4
 VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
 VR_SURFACE_ANALYTIC = 0x80000000
 VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
@@ -88,7 +88,7 @@ class vr_scene_stats(C.Structure):
                 ("num_ref_nodes", C.c_uint32), ("qbvh_depth", C.c_uint32), ("max_leaf_size", C.c_uint32),
                 ("build_ref_seconds", C.c_double), ("build_sah_seconds", C.c_double),
                 ("upload_seconds", C.c_double), ("device_bytes", C.c_uint64),
-                ("num_tri_records", C.c_uint32), ("reserved", C.c_uint32)]
+                ("num_tri_records", C.c_uint32), ("leaf_refs", C.c_uint32)]
 
 
 class vr_trace_desc(C.Structure):

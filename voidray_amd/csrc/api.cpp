@@ -173,7 +173,7 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
         if (rc) return rc;
         if (stats) {
             *stats = H.stats;
-            stats->num_tri_records = (uint32_t)H.tris.size();
+            stats->num_tri_records = H.stats.num_tri_records;
         }
         const vr::SceneView& sv = H.view;
         const bool any = desc->mode == VR_TRACE_ANY;

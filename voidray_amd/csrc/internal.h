@@ -106,6 +106,7 @@ struct GpuTexture {
 struct SceneView {
     const QNode* qnodes;
     const TriRec* tris;
+    const uint32_t* tri_ref;      // per leaf entry: index into tris (nullptr: tris are in leaf order)
     const RefNode* ref_nodes;     // all mesh trees
     const RefNode* top_nodes;     // top-level tree over surfaces
     const uint32_t* top_path;     // per-surface ancestor lists (top_nodes indices)
@@ -179,6 +180,7 @@ namespace vr {
 struct HostScene {
     std::vector<QNode> qnodes;
     std::vector<TriRec> tris;
+    std::vector<uint32_t> tri_ref;  // leaf order -> tris index; empty: tris are in leaf order
     std::vector<RefNode> ref_nodes, top_nodes;
     std::vector<uint32_t> top_path, mesh_rank, mesh_idx, prim_surface;
     std::vector<int32_t> ref_parent, prim_anc, mesh_leaf_parent;

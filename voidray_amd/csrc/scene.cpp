@@ -281,6 +281,28 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     uint32_t depth = 0;
     std::string err;
     if (build_qbvh(recs, qn, tris, depth, err) != 0) return fail(VR_ERR_INVALID, err);
+    const size_t num_leaf_entries = tris.size();
+    // Spatial splits copy a triangle into every leaf that holds a piece of it.
+    // On request leaves hold 4-byte references into one record per triangle
+    // (first-use order, so neighbouring leaves find their triangles close
+    // together): less memory, one more dependent load per leaf entry.
+    const bool refs = (d->build_flags & VR_BUILD_LEAF_REFS) && !(d->build_flags & VR_BUILD_LEAF_RECORDS);
+    H.tri_ref.clear();
+    if (refs) {
+        std::vector<uint32_t> uniq_of_prim(T, 0xFFFFFFFFu);
+        std::vector<TriRec> uniq;
+        uniq.reserve(recs.size());
+        H.tri_ref.resize(tris.size());
+        for (size_t i = 0; i < tris.size(); ++i) {
+            uint32_t& u = uniq_of_prim[tris[i].prim];
+            if (u == 0xFFFFFFFFu) {
+                u = (uint32_t)uniq.size();
+                uniq.push_back(tris[i]);
+            }
+            H.tri_ref[i] = u;
+        }
+        tris.swap(uniq);
+    }
     std::vector<TriRec>().swap(recs);
     auto t_sah1 = clk::now();
 
@@ -343,6 +365,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     // host view (the traversal code can run over it on the CPU)
     v.qnodes = qn.data();
     v.tris = tris.data();
+    v.tri_ref = H.tri_ref.empty() ? nullptr : H.tri_ref.data();
     v.ref_nodes = ref_nodes.data();
     v.top_nodes = top.data();
     v.top_path = top_path.data();
@@ -374,7 +397,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     st.num_ref_nodes = (uint32_t)ref_nodes.size();
     st.qbvh_depth = depth;
     st.max_leaf_size = kMaxLeaf;
-    st.num_tri_records = (uint32_t)tris.size();
+    st.num_tri_records = (uint32_t)num_leaf_entries;
+    st.leaf_refs = H.tri_ref.empty() ? 0u : 1u;
     st.build_ref_seconds = std::chrono::duration<double>(t_ref1 - t_ref0).count();
     st.build_sah_seconds = std::chrono::duration<double>(t_sah1 - t_ref1).count();
     return 0;
@@ -393,6 +417,7 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc = 0;
     rc |= upload(s, H.qnodes, &v.qnodes);
     rc |= upload(s, H.tris, &v.tris);
+    rc |= upload(s, H.tri_ref, &v.tri_ref);
     rc |= upload(s, H.ref_nodes, &v.ref_nodes);
     rc |= upload(s, H.top_nodes, &v.top_nodes);
     rc |= upload(s, H.top_path, &v.top_path);

@@ -282,17 +282,19 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
     uint32_t first = code & 0x07FFFFFFu;
     uint32_t cnt = ((code >> 27) & 7u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
+        // the record's index (best_slot keeps it, not the leaf entry)
+        const uint32_t slot = sv.tri_ref ? sv.tri_ref[first + k] : first + k;
         vf3 p0, p1, p2;
-        load_tri_geom(sv.tris, first + k, p0, p1, p2);
+        load_tri_geom(sv.tris, slot, p0, p1, p2);
         MTResult r;
         if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
         if (ANY) {
             if (r.t < tmax_any) {
-                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, first + k, 11),
-                                   tri_word(sv.tris, first + k, 9))) {
+                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, slot, 11),
+                                   tri_word(sv.tris, slot, 9))) {
                     st.found = true;
                     st.tbest = r.t;
-                    st.best_slot = first + k;
+                    st.best_slot = slot;
                     st.bu = r.u;
                     st.bv = r.v;
                     return kStepDone;
@@ -302,11 +304,11 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
             continue;
         }
         if (r.t > st.tbest) continue;
-        uint32_t rank = tri_word(sv.tris, first + k, 10);
+        uint32_t rank = tri_word(sv.tris, slot, 10);
         if (r.t < st.tbest || rank > st.best_rank) {
             st.tbest = r.t;
             st.best_rank = rank;
-            st.best_slot = first + k;
+            st.best_slot = slot;
             st.bu = r.u;
             st.bv = r.v;
         }
