@@ -58,11 +58,32 @@ def flat_scene():
     return sc
 
 
-def test_extended_matches_fp64_oracle_same_streams(gpu_ctx, oracle_lib):
+def analytic_scene():
+    """Analytic surfaces only (surfaces.rs): a textured ground plane, Lambertian,
+    metal and glass spheres and a glass shell around a diffuse core, HDRI sky;
+    every shading normal is the exact surface normal."""
+    from voidray_amd.scene import Surfaces
+    sc = Scene.empty()
+    alb = sc.add_image_texture(W.procedural_texture(64, 5, "albedo"))
+    sc.add_object(sc.add_material(Materials.lambertian_texture_no_normal(alb)),
+                  sc.add_analytic_surface(Surfaces.ground_plane(0.0)))
+    for c, r, m in (((-0.9, 0.4, 0.0), 0.4, Materials.lambertian((0.7, 0.4, 0.3))),
+                    ((0.1, 0.3, -0.5), 0.3, Materials.metal((0.9, 0.8, 0.6), 0.2)),
+                    ((0.8, 0.35, 0.2), 0.35, Materials.dielectric(1.5)),
+                    ((0.0, 0.5, 0.9), 0.5, Materials.dielectric(1.4)),
+                    ((0.0, 0.5, 0.9), 0.3, Materials.lambertian((0.2, 0.6, 0.3)))):
+        sc.add_object(sc.add_material(m), sc.add_analytic_surface(Surfaces.sphere(c, r)))
+    sc.environment = Environments.hdri(W.procedural_sky(64, 32, seed=4, sun_peak=60.0, sun_sharpness=30.0))
+    sc.camera = Camera.look_at((0.0, 1.3, 3.2), (0.0, 0.3, 0.0), (0.0, 1.0, 0.0), degrees_to_radians(45.0))
+    return sc
+
+
+@pytest.mark.parametrize("make_scene", [small_terrain, analytic_scene], ids=["terrain", "analytic"])
+def test_extended_matches_fp64_oracle_same_streams(make_scene, gpu_ctx, oracle_lib):
     """Same random streams: the GPU image equals the f32 restatement of the same
     estimator, and its distance to the FP64 oracle is the f32-vs-f64 rounding
     floor (paths whose decisions round differently), far below Monte Carlo noise."""
-    scene = small_terrain()
+    scene = make_scene()
     settings = RenderSettings(total_samples=64, max_bounces=16, firefly_clamp=float("inf"))
     desc = make_render_desc(32, 18, 64, 64, settings, seed=11, flags=EXT, rr_depth=4)
     ref64, _, _ = oracle_lib.OracleScene(scene, "f64").render(desc, prune=True)
@@ -82,8 +103,9 @@ def test_extended_matches_fp64_oracle_same_streams(gpu_ctx, oracle_lib):
     assert e64 <= 0.05 * noise
 
 
-def test_extended_is_unbiased_against_reference_estimator(gpu_ctx, oracle_lib):
-    scene = flat_scene()
+@pytest.mark.parametrize("make_scene", [flat_scene, analytic_scene], ids=["meshes", "analytic"])
+def test_extended_is_unbiased_against_reference_estimator(make_scene, gpu_ctx, oracle_lib):
+    scene = make_scene()
     w, h, spp = 24, 16, 1024
     plain = RenderSettings(total_samples=spp, max_bounces=12, firefly_clamp=float("inf"))
     orc = oracle_lib.OracleScene(scene, "f64")
