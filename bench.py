@@ -214,16 +214,47 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     return out
 
 
+def _render_roofline(st, env_cdf=False):
+    """HBM roofline of a render by its algorithmic bytes: per path segment 144 B
+    (ray SoA 24 B written and read, hit 16 B written and read, path state 32 B
+    read and written), 16 B per sample (RGBA accumulation), 32 B per textured
+    hit, 48 B per environment miss and, with environment importance sampling,
+    8 B per CDF lookup; counts from the run's own counters."""
+    b = 144 * st["segments"] + 16 * st["samples"] + 32 * st["textured_hits"] + 48 * st["env_misses"]
+    if env_cdf:
+        b += 8 * st["env_is_samples"]
+    gbs = b / (st["kernel_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 2), "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 5),
+            "algorithmic_bytes": int(b), "textured_hits": st["textured_hits"], "env_misses": st["env_misses"],
+            "env_is_samples": st["env_is_samples"]}
+
+
+def _precision(img, ref, rows=None):
+    """Max per-pixel relative error (floor |r| >= 1e-3) and relMSE of img against ref."""
+    import numpy as np
+    x = img[..., :3].astype(np.float64)
+    r = ref[..., :3].astype(np.float64)
+    if rows is not None:
+        x, r = x[rows], r[rows]
+    return {"max_rel_err": float(np.max(np.abs(x - r) / np.maximum(np.abs(r), 1e-3))),
+            "relmse": float(np.mean((x - r) ** 2 / (r * r + 0.01))),
+            "bit_identical_fraction": round(float(np.mean(x == r)), 4)}
+
+
 def _render_legs(ctx, cpu=True, c3_spp=64):
     """Path-integration legs reported beside the headline metric:
     * cornell 512x512, 16 spp, depth 8, reference estimator (firefly clamp 3);
-    * terrain+spheres 1920x1080, 64 spp, depth 16, RR from 4, env IS + MIS.
-    Device time of vr_render (HIP events), samples/s and segments (rays)/s."""
+    * terrain+spheres 1920x1080, 64 spp, depth 16, RR from 4, env IS + MIS,
+      and the same frame with the reference estimator (no RR, no env IS,
+      firefly clamp 3).
+    Device time of vr_render (HIP events), samples/s, segments (rays)/s, the
+    render's HBM roofline by algorithmic bytes, and, with the CPU legs, the
+    oracle timed on the host and the GPU image's error against it."""
     import numpy as np
 
     from voidray_amd import _native as N
-    from voidray_amd import workloads as W
     from voidray_amd.device import make_render_desc
+    from voidray_amd import workloads as W
     from voidray_amd.scene import RenderSettings
     out = {}
     c1 = W.cornell_room()
@@ -232,11 +263,21 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
     img = np.zeros((512, 512, 4), np.float32)
     a1.render(img, make_render_desc(512, 512, 1, 16, s1, seed=99))  # warm-up
     img[:] = 0
-    st = a1.render(img, make_render_desc(512, 512, 16, 16, s1, seed=1))
-    out["cornell_512_16spp"] = {"ms": round(st["kernel_ms"], 3),
-                                "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
-                                "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2),
-                                "segments": st["segments"], "exact_engine_rays": st["fallback_rays"]}
+    d1 = make_render_desc(512, 512, 16, 16, s1, seed=1)
+    st = a1.render(img, d1)
+    leg = {"ms": round(st["kernel_ms"], 3), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
+           "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
+           "exact_engine_rays": st["fallback_rays"], "roofline": _render_roofline(st)}
+    if cpu:
+        import oracle
+        threads = min(16, oracle.num_threads())
+        ref, sec, _ = oracle.OracleScene(c1).render(d1, threads=threads)
+        n = 512 * 512 * 16
+        leg["cpu_baseline"] = {"samples_per_s": round(n / sec, 1), "cores": threads, "kind": "port",
+                               "sample": "the full frame (512x512, 16 spp), reference estimator and traversal"}
+        leg["gpu_over_cpu"] = round(leg["samples_per_s"] / (n / sec), 1)
+        leg["precision_vs_f32_restatement"] = _precision(img, ref)
+    out["cornell_512_16spp"] = leg
     t0 = time.time()
     c3, a3 = _terrain(ctx)
     build3 = time.time() - t0
@@ -248,19 +289,38 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
     st = a3.render(img3, make_render_desc(1920, 1080, c3_spp, c3_spp, s3, seed=1, flags=ext, rr_depth=4))
     leg = {"ms": round(st["kernel_ms"], 2), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
            "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
-           "exact_engine_rays": st["fallback_rays"], "triangles": c3.num_triangles, "scene_build_s": round(build3, 2)}
+           "exact_engine_rays": st["fallback_rays"], "triangles": c3.num_triangles, "scene_build_s": round(build3, 2),
+           "roofline": _render_roofline(st, env_cdf=True)}
     if cpu:
         import oracle
         threads = min(16, oracle.num_threads())
         orc = oracle.OracleScene(c3)
         # bounded sample: 1 spp on every 3rd row (360 rows x 1920 pixels), same estimator, reference traversal
         d1 = make_render_desc(1920, 1080, 1, c3_spp, s3, seed=5, flags=ext, rr_depth=4)
-        _, sec, ost = orc.render(d1, threads=threads, row_step=3)
+        ref, sec, ost = orc.render(d1, threads=threads, row_step=3)
         n = 1920 * 360
         leg["cpu_baseline"] = {"samples_per_s": round(n / sec, 1), "cores": threads, "kind": "port",
                                "sample": f"{n} samples (1 spp, every 3rd row), reference median-split traversal"}
         leg["gpu_over_cpu"] = round(leg["samples_per_s"] / (n / sec), 1)
+        # precision: the same 1-spp frame on the GPU against the f32 restatement on
+        # those rows, and against the FP64 oracle on every 30th row
+        g1 = np.zeros((1080, 1920, 4), np.float32)
+        a3.render(g1, d1)
+        leg["precision_vs_f32_restatement"] = _precision(g1, ref, rows=slice(0, None, 3))
+        ref64, _, _ = oracle.OracleScene(c3, "f64").render(d1, threads=threads, row_step=30, prune=True)
+        leg["precision_vs_fp64_oracle"] = _precision(g1, ref64, rows=slice(0, None, 30))
+        leg["precision_note"] = "1 spp, seed 5, same random streams; f32 rows 0,3,6,...; FP64 rows 0,30,60,..."
     out["terrain_1080p_64spp"] = leg
+    # the same frame under the reference estimator (no RR, no env IS, clamp 3)
+    sr = RenderSettings(total_samples=c3_spp, max_bounces=16)
+    imr = np.zeros((1080, 1920, 4), np.float32)
+    a3.render(imr, make_render_desc(1920, 1080, 1, c3_spp, sr, seed=98))  # warm-up
+    imr[:] = 0
+    st = a3.render(imr, make_render_desc(1920, 1080, c3_spp, c3_spp, sr, seed=1))
+    out["terrain_1080p_64spp_reference_estimator"] = {
+        "ms": round(st["kernel_ms"], 2), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
+        "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
+        "exact_engine_rays": st["fallback_rays"], "roofline": _render_roofline(st)}
     return out
 
 

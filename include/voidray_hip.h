@@ -283,6 +283,8 @@ typedef struct vr_render_stats {
     uint64_t fallback_rays;   /* rays re-traced by the exact engine */
     uint64_t capped_loops;    /* rejection loops that hit their cap */
     double kernel_ms;         /* device time of the render passes */
+    uint64_t textured_hits;   /* hits on a material with an albedo or normal texture */
+    uint64_t env_is_samples;  /* environment importance samples (extended estimator) */
 } vr_render_stats;
 
 /* Adds (sum over `samples` of L) * (1/total_samples) into rgba_accum (host,

@@ -98,6 +98,8 @@ def test_extended_matches_fp64_oracle_same_streams(make_scene, gpu_ctx, oracle_l
     print(f"[extended] relMSE vs FP64 = {e64:.3e} (f32-vs-f64 floor {floor:.3e}, independent-seed noise "
           f"{noise:.3e}); vs f32 restatement = {e32:.3e}; segments {st['segments']}", flush=True)
     assert np.isfinite(img).all()
+    assert st["env_is_samples"] > 0 and st["env_is_samples"] <= st["segments"]
+    assert st["textured_hits"] > 0
     assert e32 <= 1e-7
     assert e64 <= 1.5 * floor + 1e-7
     assert e64 <= 0.05 * noise

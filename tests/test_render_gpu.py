@@ -67,6 +67,9 @@ def test_render_matches_oracle(case, gpu_ctx, oracle_lib):
     assert err <= TOL, f"{case}: max relative pixel error {err}"
     assert stats["samples"] == w * h * spp
     assert stats["segments"] == ost["segments"], (stats["segments"], ost["segments"])
+    assert stats["env_is_samples"] == 0  # reference estimator: no environment sampling
+    has_tex = any(m.albedo_texture >= 0 or m.normal_texture >= 0 for m in scene.materials)
+    assert (stats["textured_hits"] > 0) == has_tex
     assert float(np.abs(ref[..., :3]).sum()) > 0
 
 

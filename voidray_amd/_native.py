@@ -104,7 +104,8 @@ class vr_render_desc(C.Structure):
 
 class vr_render_stats(C.Structure):
     _fields_ = [("segments", C.c_uint64), ("samples", C.c_uint64), ("env_misses", C.c_uint64),
-                ("fallback_rays", C.c_uint64), ("capped_loops", C.c_uint64), ("kernel_ms", C.c_double)]
+                ("fallback_rays", C.c_uint64), ("capped_loops", C.c_uint64), ("kernel_ms", C.c_double),
+                ("textured_hits", C.c_uint64), ("env_is_samples", C.c_uint64)]
 
 
 class vr_post_desc(C.Structure):

@@ -51,9 +51,17 @@ static inline uint32_t grid_for(uint32_t n, uint32_t block) {
 enum : int {
     kCntActive = 0,                                  // [kMaxBouncesLimit + 1] live paths per depth
     kCntShadow = kCntActive + kMaxBouncesLimit + 1,  // [kMaxBouncesLimit] shadow rays per depth
-    kCntMiss = kCntShadow + kMaxBouncesLimit,
-    kCntCapped,
-    kCntWords
+    kCntCapped = kCntShadow + kMaxBouncesLimit,
+    // counters every shading wave adds to, striped over kStatStripes 128-byte
+    // lines (stripe = block index mod kStatStripes) so that the waves' atomics
+    // do not all meet on one address; k_stats_chunk sums the stripes
+    kCntStriped = (kCntCapped + 1 + 31) & ~31,
+    kStatStripes = 64,
+    kStripeWords = 32,
+    kStripeMiss = 0,      // environment misses
+    kStripeTextured = 1,  // hits on a material with a texture
+    kStripeEnvIS = 2,     // environment importance samples (CDF lookups) for next-event estimation
+    kCntWords = kCntStriped + kStatStripes * kStripeWords
 };
 
 // radiance-cache mode of a request: paths stop at the query vertex and take
@@ -245,7 +253,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
         uint32_t k = kb + (threadIdx.x & 63u);
         bool act = k < n;
         uint32_t p = 0;
-        bool cont = false, shadow = false, miss = false;
+        bool cont = false, shadow = false, miss = false, textured = false, env_is = false;
         uint32_t rec = 0;  // radiance cache: 1 = training sample, 2 = query
         vf3 rec_pos{}, rec_dir{}, rec_n{}, rec_A{}, rec_L{};
         float rec_rough = 0.f;
@@ -280,6 +288,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
             } else {
                 HitRec hit = make_hit(sv, o, d, prim, a.hit_t[p], a.hit_u[p], a.hit_v[p]);
                 GpuMaterial m = sv.materials[sv.surfaces[hit.surface].material];
+                textured = m.albedo_tex >= 0 || m.normal_tex >= 0;
                 if (!r.ext) {
                     if (r.mode == VR_MODE_NORMAL) {
                         vf3 nn = normalize3(hit.normal);
@@ -330,6 +339,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                             ns = normalize3(ns);
                             float pe = 0.f;
                             vf3 wi = env_sample_dir(sv, u1, u2, pe);
+                            env_is = true;
                             float cw = dot3(ns, wi);
                             if (cw > 0.f && pe > 0.f) {
                                 float pb = cw / kPi;
@@ -387,7 +397,14 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
         wave_append(cont, p, qout, a.cnt + kCntActive + depth + 1);
         wave_append(shadow, p, a.sh_queue, a.cnt + kCntShadow + depth);
         uint64_t mm = __ballot(miss);
-        if ((threadIdx.x & 63u) == 0 && mm) atomicAdd(a.cnt + kCntMiss, (uint32_t)__popcll(mm));
+        uint64_t tm = __ballot(textured);
+        uint64_t em = __ballot(env_is);
+        if ((threadIdx.x & 63u) == 0) {
+            uint32_t* stripe = a.cnt + kCntStriped + (blockIdx.x % kStatStripes) * kStripeWords;
+            if (mm) atomicAdd(stripe + kStripeMiss, (uint32_t)__popcll(mm));
+            if (tm) atomicAdd(stripe + kStripeTextured, (uint32_t)__popcll(tm));
+            if (em) atomicAdd(stripe + kStripeEnvIS, (uint32_t)__popcll(em));
+        }
         if (capped) atomicAdd(a.cnt + kCntCapped, capped);
     }
 }
@@ -461,7 +478,7 @@ __global__ void k_accumulate(RenderArgs a) {
 }
 
 // Adds one chunk's counters into the 64-bit statistics
-// [segments, samples, env misses, exact-engine rays, capped loops].
+// [segments, samples, env misses, exact-engine rays, capped loops, textured hits, env IS samples].
 __global__ void k_stats_chunk(const uint32_t* cnt, uint64_t* stats, uint32_t npaths, uint32_t max_depth) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint64_t seg = max_depth ? npaths : 0;
@@ -469,8 +486,17 @@ __global__ void k_stats_chunk(const uint32_t* cnt, uint64_t* stats, uint32_t npa
     for (uint32_t k = 0; k < max_depth; ++k) seg += cnt[kCntShadow + k];
     stats[0] += seg;
     stats[1] += npaths;
-    stats[2] += cnt[kCntMiss];
+    uint64_t miss = 0, tex = 0, env = 0;
+    for (int s = 0; s < kStatStripes; ++s) {
+        const uint32_t* stripe = cnt + kCntStriped + s * kStripeWords;
+        miss += stripe[kStripeMiss];
+        tex += stripe[kStripeTextured];
+        env += stripe[kStripeEnvIS];
+    }
+    stats[2] += miss;
     stats[4] += cnt[kCntCapped];
+    stats[5] += tex;
+    stats[6] += env;
 }
 
 __global__ void k_stats_zero(uint64_t* stats) {
@@ -1039,6 +1065,8 @@ int render_finish_stats(vr_ctx* ctx, vr_render_stats* stats) {
     stats->env_misses = h[2];
     stats->fallback_rays = h[3];
     stats->capped_loops = h[4];
+    stats->textured_hits = h[5];
+    stats->env_is_samples = h[6];
     return 0;
 }
 
