@@ -134,28 +134,69 @@ __device__ __forceinline__ uint32_t seg_find(const SegDev* s, uint32_t n, uint32
     return lo;
 }
 
-// Appends `want` lanes to a queue: one atomic per wave.
-__device__ __forceinline__ void wave_append(bool want, uint32_t value, uint32_t* queue, uint32_t* counter) {
-    uint64_t mask = __ballot(want);
-    if (mask == 0) return;
-    int lane = threadIdx.x & 63;
-    int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (want) queue[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+// Appends to two queues with one atomic per queue per block (every wave of
+// the block must call it, the same number of times): the waves' counts meet
+// in LDS, one lane adds the block's total to each counter, and each wave
+// writes at the block's base plus the counts of the waves before it.
+// Every shading wave appending to the same two counters directly made those
+// two addresses a serialisation point.
+__device__ __forceinline__ void block_append2(bool want0, bool want1, uint32_t value, uint32_t* q0, uint32_t* c0,
+                                              uint32_t* q1, uint32_t* c1) {
+    __shared__ uint32_t s_n[2][16];
+    __shared__ uint32_t s_base[2];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    const uint64_t m0 = __ballot(want0), m1 = __ballot(want1);
+    if (lane == 0) {
+        s_n[0][wave] = (uint32_t)__popcll(m0);
+        s_n[1][wave] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) tot += s_n[threadIdx.x][w];
+        s_base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? c1 : c0, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off0 = s_base[0], off1 = s_base[1];
+    for (uint32_t w = 0; w < wave; ++w) {
+        off0 += s_n[0][w];
+        off1 += s_n[1][w];
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (want0) q0[off0 + (uint32_t)__popcll(m0 & below)] = value;
+    if (want1) q1[off1 + (uint32_t)__popcll(m1 & below)] = value;
+    __syncthreads();  // s_n and s_base are rewritten by the next call
 }
 
-// Slot index for each lane that wants one (one atomic per wave); UINT32_MAX otherwise.
-__device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
-    uint64_t mask = __ballot(want);
-    if (mask == 0) return 0xFFFFFFFFu;
-    int lane = threadIdx.x & 63;
-    int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return want ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
+// Slots in two sample buffers for the lanes that want one, one atomic per
+// buffer per block (every wave of the block must call it, the same number of
+// times); UINT32_MAX for lanes that want none.
+__device__ __forceinline__ void block_alloc2(bool want0, bool want1, uint32_t* c0, uint32_t* c1, uint32_t& slot0,
+                                             uint32_t& slot1) {
+    __shared__ uint32_t s_n[2][16];
+    __shared__ uint32_t s_base[2];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    const uint64_t m0 = __ballot(want0), m1 = __ballot(want1);
+    if (lane == 0) {
+        s_n[0][wave] = (uint32_t)__popcll(m0);
+        s_n[1][wave] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) tot += s_n[threadIdx.x][w];
+        s_base[threadIdx.x] = tot ? atomicAdd(threadIdx.x ? c1 : c0, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off0 = s_base[0], off1 = s_base[1];
+    for (uint32_t w = 0; w < wave; ++w) {
+        off0 += s_n[0][w];
+        off1 += s_n[1][w];
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    slot0 = want0 ? off0 + (uint32_t)__popcll(m0 & below) : 0xFFFFFFFFu;
+    slot1 = want1 ? off1 + (uint32_t)__popcll(m1 & below) : 0xFFFFFFFFu;
+    __syncthreads();  // s_n and s_base are rewritten by the next call
 }
 
 // Training-path selection: a 32-bit mix of (pixel, sample, request, frame)
@@ -247,10 +288,10 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
     const uint32_t* qin = depth == 0 ? nullptr : a.queue[depth & 1];
     uint32_t* qout = a.queue[(depth + 1) & 1];
     const uint32_t stride = gridDim.x * blockDim.x;
-    // uniform trip count per wave so that every lane reaches the wave-wide appends
-    uint32_t k0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t kb = k0; kb < n; kb += stride) {
-        uint32_t k = kb + (threadIdx.x & 63u);
+    // the same trip count for every thread of the block, so that all of them
+    // reach the block-wide appends
+    for (uint32_t kb = blockIdx.x * blockDim.x; kb < n; kb += stride) {
+        uint32_t k = kb + threadIdx.x;
         bool act = k < n;
         uint32_t p = 0;
         bool cont = false, shadow = false, miss = false, textured = false, env_is = false;
@@ -378,8 +419,9 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
             a.rng_n[p] = rng.n;
         }
         {
-            const uint32_t slot_t = wave_alloc(rec == 1u, a.tr.count);
-            const uint32_t slot_q = wave_alloc(rec == 2u, a.qr.count);
+            uint32_t slot_t = 0xFFFFFFFFu, slot_q = 0xFFFFFFFFu;
+            if (a.tr.count)  // the same for the whole grid: a batch with radiance-cache requests
+                block_alloc2(rec == 1u, rec == 2u, a.tr.count, a.qr.count, slot_t, slot_q);
             if (rec) {
                 const bool t = rec == 1u;
                 const uint32_t slot = t ? slot_t : slot_q;
@@ -394,8 +436,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                 }
             }
         }
-        wave_append(cont, p, qout, a.cnt + kCntActive + depth + 1);
-        wave_append(shadow, p, a.sh_queue, a.cnt + kCntShadow + depth);
+        block_append2(cont, shadow, p, qout, a.cnt + kCntActive + depth + 1, a.sh_queue, a.cnt + kCntShadow + depth);
         uint64_t mm = __ballot(miss);
         uint64_t tm = __ballot(textured);
         uint64_t em = __ballot(env_is);
