@@ -37,7 +37,14 @@
 
 namespace vr {
 
-constexpr uint32_t kDefaultChunkPaths = 1u << 22;
+// Paths per wavefront chunk.  Every bounce of a chunk is a few dependent
+// launches over the chunk's live paths; once Russian roulette and escapes
+// thin them out, a small chunk leaves the traversal kernels' persistent waves
+// mostly idle.  On the 1080p x 64 spp terrain frame: 2^22 paths per chunk
+// 394 ms, 2^24 215 ms, 2^26 166 ms (about 133 B of path state per path with
+// the extended estimator, 272 B with the reference estimator's 16-level
+// attenuation stack: 9 or 18 GB of the GPU's 288 GB at 2^26).
+constexpr uint32_t kDefaultChunkPaths = 1u << 26;
 constexpr uint32_t kMaxChunkPaths = 1u << 26;
 constexpr uint32_t kMaxBouncesLimit = 64;
 constexpr uint32_t kShadeGrid = 2048;  // most blocks a grid-stride kernel over a chunk launches
@@ -1129,7 +1136,7 @@ int batch_latencies(vr_ctx* ctx, double* ms, uint32_t n) {
 // relaunches after one warm-up launch) and returns the fastest.
 int autotune_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, uint32_t flags, uint32_t reps,
                    hipStream_t st, vr_tuning* best, float* best_ms) {
-    static const uint32_t chunks[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    static const uint32_t chunks[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
     static const uint32_t blocks[] = {128, 256, 512};
     uint32_t max_spp = 1;
     for (uint32_t i = 0; i < n; ++i) max_spp = std::max(max_spp, items[i].desc.samples);
