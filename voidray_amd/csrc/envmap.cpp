@@ -78,4 +78,19 @@ double build_env_tables(const float* rgb, unsigned W, unsigned H, std::vector<fl
     return total;
 }
 
+uint32_t guide_size(uint32_t n) {
+    uint32_t g = 1;
+    while (g < n) g <<= 1;
+    return g;
+}
+
+void build_guide(const float* cdf, uint32_t n, uint32_t G, uint32_t* guide) {
+    uint32_t k = 0;
+    for (uint32_t j = 0; j <= G; ++j) {
+        const float t = (float)j / (float)G;  // exact: G is a power of two
+        while (k + 1 < n && cdf[k + 1] <= t) ++k;
+        guide[j] = k;
+    }
+}
+
 }  // namespace vr

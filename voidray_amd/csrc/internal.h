@@ -130,6 +130,9 @@ struct SceneView {
     const float* env_marg_cdf;    // env importance sampling tables (f32 copies of FP64 prefix sums)
     const float* env_cond_cdf;
     const float* env_pdf;         // per texel pdf (solid-angle measure factor folded in at use)
+    const uint32_t* env_marg_guide;  // guide tables of the CDFs: marginal (env_gm + 1 entries) and
+    const uint32_t* env_cond_guide;  // per row (env_gc + 1 entries each); see build_guide
+    uint32_t env_gm, env_gc;
     uint32_t num_qnodes, num_tris, num_surfaces, top_root_valid;
     int32_t env_kind;
     float env_color[3];
@@ -188,6 +191,7 @@ struct HostScene {
     std::vector<SurfaceInfo> surfaces;
     std::vector<AnalyticDev> analytic;
     std::vector<float> face_n, vpos, vnrm, vuv, env, env_marg, env_cond, env_pdf;
+    std::vector<uint32_t> env_marg_guide, env_cond_guide;
     std::vector<GpuMaterial> materials;
     std::vector<GpuTexture> textures;
     std::vector<uint8_t> tex_data;

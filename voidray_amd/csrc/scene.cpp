@@ -338,6 +338,14 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
         if (!(d->build_flags & VR_BUILD_NO_ENV_CDF)) {
             double total = build_env_tables(env.data(), e.width, e.height, env_marg, env_cond, env_pdf);
             v.env_total = (float)total;
+            v.env_gm = guide_size(e.height);
+            v.env_gc = guide_size(e.width);
+            H.env_marg_guide.resize(v.env_gm + 1);
+            build_guide(env_marg.data(), e.height, v.env_gm, H.env_marg_guide.data());
+            H.env_cond_guide.resize((size_t)e.height * (v.env_gc + 1));
+            for (uint32_t r = 0; r < e.height; ++r)
+                build_guide(env_cond.data() + (size_t)r * (e.width + 1), e.width, v.env_gc,
+                            H.env_cond_guide.data() + (size_t)r * (v.env_gc + 1));
         }
     } else if (v.env_kind != VR_ENV_NONE && v.env_kind != VR_ENV_UNIFORM) {
         return fail(VR_ERR_INVALID, "unknown environment kind");
@@ -388,6 +396,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.env = env.data();
     v.env_marg_cdf = env_marg.data();
     v.env_cond_cdf = env_cond.data();
+    v.env_marg_guide = H.env_marg_guide.empty() ? nullptr : H.env_marg_guide.data();
+    v.env_cond_guide = H.env_cond_guide.empty() ? nullptr : H.env_cond_guide.data();
     v.env_pdf = env_pdf.data();
 
     vr_scene_stats& st = H.stats;
@@ -439,6 +449,8 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.env, &v.env);
     rc |= upload(s, H.env_marg, &v.env_marg_cdf);
     rc |= upload(s, H.env_cond, &v.env_cond_cdf);
+    rc |= upload(s, H.env_marg_guide, &v.env_marg_guide);
+    rc |= upload(s, H.env_cond_guide, &v.env_cond_guide);
     rc |= upload(s, H.env_pdf, &v.env_pdf);
     if (rc) return VR_ERR_OOM;
     s->num_prims = v.num_tris;

@@ -442,6 +442,21 @@ VR_HD uint32_t cdf_search(const float* cdf, uint32_t n, float u) {
     return lo;
 }
 
+// The same cell as cdf_search(cdf, n, u), searched only between the guide
+// table's bounds for u (G a power of two, so floor(u * G) is exact).
+VR_HD uint32_t cdf_search_guided(const float* cdf, const uint32_t* guide, uint32_t n, uint32_t G, float u) {
+    uint32_t j = (uint32_t)(u * (float)G);
+    j = j < G ? j : G - 1u;
+    uint32_t lo = guide[j], hi = guide[j + 1] + 1u;
+    hi = hi < n ? hi : n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 VR_HD float env_pdf_omega(const SceneView& sv, float x, float y, float sin_t) {
     if (!(sin_t > 0.f)) return 0.f;
     int64_t i = (int64_t)floorf(x);
@@ -466,10 +481,10 @@ VR_HD float env_pdf_dir(const SceneView& sv, vf3 dir) {
 
 VR_HD vf3 env_sample_dir(const SceneView& sv, float u1, float u2, float& pdf) {
     uint32_t H = sv.env_h, W = sv.env_w;
-    uint32_t r = cdf_search(sv.env_marg_cdf, H, u1);
+    uint32_t r = cdf_search_guided(sv.env_marg_cdf, sv.env_marg_guide, H, sv.env_gm, u1);
     float m0 = sv.env_marg_cdf[r], m1 = sv.env_marg_cdf[r + 1];
     const float* c = sv.env_cond_cdf + (size_t)r * (W + 1);
-    uint32_t i = cdf_search(c, W, u2);
+    uint32_t i = cdf_search_guided(c, sv.env_cond_guide + (size_t)r * (sv.env_gc + 1), W, sv.env_gc, u2);
     float c0 = c[i], c1 = c[i + 1];
     float fx = c1 > c0 ? (u2 - c0) / (c1 - c0) : 0.5f;
     float fy = m1 > m0 ? (u1 - m0) / (m1 - m0) : 0.5f;
