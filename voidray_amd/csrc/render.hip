@@ -10,7 +10,7 @@
 //   for depth = 0 .. max over the chunk's requests of max_bounces - 1:
 //     extend   closest hit of every live path (wide engine + exact fallback)
 //     shade    hit record, material scatter, termination; survivors are
-//              appended to the next queue with one atomic per wave
+//              appended to the next queue with one atomic per block
 //     shadow   (extended estimator) any-hit test of environment samples
 //   accumulate  per pixel, samples summed in index order, * 1/total_samples,
 //              added into the request's RGBA f32 target (iterative.rs:45-51)
