@@ -46,6 +46,10 @@ namespace vr {
 // attenuation stack: 9 or 18 GB of the GPU's 288 GB at 2^26).
 constexpr uint32_t kDefaultChunkPaths = 1u << 26;
 constexpr uint32_t kMaxChunkPaths = 1u << 26;
+// The default chunk is halved until the batch's path state fits this budget
+// (the reference estimator's attenuation stack grows with max_bounces: at the
+// 64-bounce limit a 2^26-path chunk would hold 55 GB).
+constexpr size_t kDefaultScratchBudget = 24ull << 30;
 constexpr uint32_t kMaxBouncesLimit = 64;
 constexpr uint32_t kShadeGrid = 2048;  // most blocks a grid-stride kernel over a chunk launches
 
@@ -763,6 +767,18 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         g.max_depth = std::max(g.max_depth, d.max_bounces);
         g.paths += rp;
         g.any_cached |= r.cmode == kCacheOn;
+    }
+    if (!tune.chunk_paths) {
+        uint32_t att = 0, max_spp = 1;
+        bool ext = false;
+        for (const Group& g : R.groups) {
+            att = std::max(att, g.att_levels);
+            ext |= g.any_ext;
+        }
+        for (const ReqDev& r : R.reqs) max_spp = std::max(max_spp, r.spp);
+        while (chunk_paths > 1u && chunk_paths / 2 >= max_spp && scratch_bytes(chunk_paths, att, ext) > kDefaultScratchBudget)
+            chunk_paths /= 2;
+        R.tune.chunk_paths = chunk_paths;
     }
     if (flags & VR_BATCH_SHORTEST_FIRST) {
         std::stable_sort(R.groups.begin(), R.groups.end(),
