@@ -156,11 +156,14 @@ VR_HD vf2 unit_circle(PathRng& rng, uint32_t& capped) {
 // for results in the denormal range (x < about -87.3), where glibc returns the
 // denormal; the Beckmann pdf reaches that range, and 0 there turns a sample
 // the reference keeps (tiny pdf, clamped firefly) into "no scatter".
-VR_HD float exp_libm(float x) { return (float)exp((double)x); }
-VR_HD float ln_libm(float x) { return (float)log((double)x); }
-VR_HD float sin_libm(float x) { return (float)sin((double)x); }
-VR_HD float cos_libm(float x) { return (float)cos((double)x); }
-VR_HD float atan_libm(float x) { return (float)atan((double)x); }
+// Not inlined: the f64 routines are large, and inlined into the shading
+// kernel their registers made every path spill (344 bytes of scratch per
+// lane; 64 with calls), though only microfacet hits run them.
+__noinline__ VR_HD float exp_libm(float x) { return (float)exp((double)x); }
+__noinline__ VR_HD float ln_libm(float x) { return (float)log((double)x); }
+__noinline__ VR_HD float sin_libm(float x) { return (float)sin((double)x); }
+__noinline__ VR_HD float cos_libm(float x) { return (float)cos((double)x); }
+__noinline__ VR_HD float atan_libm(float x) { return (float)atan((double)x); }
 
 // Beckmann distribution term D (microfacet.rs:140-143, 183-186).
 VR_HD float beckmann_d(float m2, float nh2) { return exp_libm((nh2 - 1.0f) / (m2 * nh2)) / (((m2 * kPi) * nh2) * nh2); }
