@@ -157,3 +157,32 @@ def test_tonemap_matches_oracle(mode, gpu_ctx, oracle_lib):
     # GLSL pow of a negative base is undefined; both sides give NaN at the same pixels
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     assert np.allclose(got, ref, rtol=2e-6, atol=1e-7, equal_nan=True), float(np.nanmax(np.abs(got - ref)))
+
+
+def test_scene_from_files(tmp_path, gpu_ctx, oracle_lib):
+    """A mesh and a texture written to disk and read back the way the reference
+    reads its assets (Mesh::from_file with obj-rs, image::open(..).to_rgb32f()),
+    rendered on the GPU against the oracle."""
+    from PIL import Image
+
+    from voidray_amd.scene import Camera, Environments, Materials, SampleType, Scene, Surfaces
+    sph = W.uv_sphere((0.0, 0.5, 0.0), 0.5, 24, 12)
+    lines = [f"v {x:.6f} {y:.6f} {z:.6f}" for x, y, z in sph.positions]
+    lines += [f"vt {u:.6f} {v:.6f}" for u, v in sph.uvs]
+    lines += [f"vn {x:.6f} {y:.6f} {z:.6f}" for x, y, z in sph.normals]
+    lines += ["f " + " ".join(f"{i + 1}/{i + 1}/{i + 1}" for i in tri) for tri in sph.indices]
+    (tmp_path / "sphere.obj").write_text("\n".join(lines) + "\n")
+    tex = (W.procedural_texture(64, 9, "albedo")[..., :3]).astype(np.uint8)
+    Image.fromarray(tex, "RGB").save(tmp_path / "albedo.png")
+    sc = Scene.empty()
+    surf = sc.add_mesh_from_file(str(tmp_path / "sphere.obj"))
+    t = sc.add_image_texture(str(tmp_path / "albedo.png"), SampleType.Bilinear)
+    sc.add_object(sc.add_material(Materials.lambertian_texture_no_normal(t)), surf)
+    sc.add_object(sc.add_material(Materials.lambertian((0.5, 0.5, 0.5))), sc.add_analytic_surface(Surfaces.ground_plane(0.0)))
+    sc.camera = Camera.look_at((0.0, 1.0, 2.5), (0.0, 0.4, 0.0), (0.0, 1.0, 0.0), 0.8)
+    sc.environment = Environments.hdri(W.procedural_sky(64, 32, sun_peak=50.0, sun_sharpness=40.0))
+    assert sc.meshes[0].num_triangles == len(sph.indices)
+    announce("render scene read from OBJ + PNG files")
+    img, ref, stats, ost = _render_pair(sc, gpu_ctx, oracle_lib, 48, 40, 4, RenderSettings(total_samples=4, max_bounces=8))
+    assert _rel_err(img, ref) <= TOL
+    assert stats["textured_hits"] > 0
