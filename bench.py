@@ -333,8 +333,9 @@ def _serving_leg(ctx, torch, stream):
     """64 render requests (resolutions 256^2..1920x1080, 1-32 spp, own seeds)
     over eight scenes of 10k-2M triangles; depth 16, RR from 4, env IS + MIS.
     Batched: one captured hipGraph launch with the autotuned chunk/block size
-    (tuned on a proxy with spp / 8); completion time of each request from
-    device time stamps.  Sequential: the same requests one render call each,
+    (tuned on a proxy with spp / 8, the pick confirmed against the largest
+    chunk on the real batch); completion time of each request from device
+    time stamps.  Sequential: the same requests one render call each,
     shortest first, back to back on the same stream; completion times from HIP
     events.  Latency = completion time with all requests submitted at t = 0.
     The per-image checksums of both runs must agree (same random streams)."""
@@ -365,6 +366,21 @@ def _serving_leg(ctx, torch, stream):
     br = BatchRenderer(ctx)
     t1 = time.time()
     (chunk, block), tune_ms = br.autotune(proxy, reps=1, shortest_first=True, stream=sp)
+    # the proxy has 8x fewer paths, so it cannot see what the largest chunks
+    # gain on the real batch: the proxy's pick and the same block at the
+    # largest chunk are each run once on the real batch, and the faster one kept
+    confirm = {}
+    for cand in dict.fromkeys([(chunk, block), (1 << 26, block)]):
+        br.submit(batch, tuning=cand, graph=True, shortest_first=True, stream=sp)
+        br.stats()
+        c0 = torch.cuda.Event(enable_timing=True)
+        c1 = torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        br.relaunch(stream=sp)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        confirm[cand] = c0.elapsed_time(c1)
+    chunk, block = min(confirm, key=confirm.get)
     tune_s = time.time() - t1
     # batched: capture + first launch (warm-up), then one timed relaunch
     t1 = time.time()
@@ -399,7 +415,8 @@ def _serving_leg(ctx, torch, stream):
         "requests": len(reqs), "samples": samples, "scenes_triangles": [s.num_triangles for s in pool],
         "scene_build_s": round(build_s, 2), "settings": "depth 16, RR from 4, env IS + MIS",
         "autotune": {"chunk_paths": chunk, "block": block, "proxy_ms": round(tune_ms, 2),
-                     "seconds": round(tune_s, 1), "proxy": "same requests with spp // 8"},
+                     "seconds": round(tune_s, 1), "proxy": "same requests with spp // 8",
+                     "confirmed_on_batch_ms": {f"{c}x{b}": round(v, 1) for (c, b), v in confirm.items()}},
         "batched": {"ms": round(batch_ms, 2), "samples_per_s": round(samples / (batch_ms * 1e-3), 1),
                     "mrays_per_s": round(bst["segments"] / (batch_ms * 1e3), 2),
                     "p50_ms": _pct(lat_b, 50), "p99_ms": _pct(lat_b, 99),
