@@ -1006,6 +1006,8 @@ V3<R> trace_ext(const Scene<R>& s, const EnvTables<R>* T, const vr_render_desc& 
         Scatter<R> sc = scatter(s, m, ray, hit, rng, rc, true, &lobe);
         if (!sc.has_ray || depth + 1 >= d.max_bounces) break;
         A = mulv(A, sc.att);
+        // a non-finite throughput (0 * inf: a zero BSDF value over an underflowed pdf) ends the path
+        if (!std::isfinite(A.x) || !std::isfinite(A.y) || !std::isfinite(A.z)) break;
         R pb = m.kind == VR_MAT_LAMBERTIAN ? std::max(R(0), dot(lobe, sc.ray.d)) / pi_v<R>() : R(0);
         if ((d.flags & VR_RENDER_RUSSIAN_ROULETTE) && depth >= d.rr_depth) {
             R q = std::min(R(0.95f), std::max(A.x, std::max(A.y, A.z)));

@@ -124,3 +124,25 @@ def test_extended_is_unbiased_against_reference_estimator(make_scene, gpu_ctx, o
           f"means {m_img:.5f} / {m_ref:.5f}", flush=True)
     assert e <= 2.0 * noise + 1e-6
     assert abs(m_img - m_ref) <= 0.02 * m_ref
+
+
+def test_extended_microfacet_same_streams(gpu_ctx, oracle_lib):
+    """MicrofacetBSDF (microfacet.rs) under the extended estimator, same random
+    streams: every pixel within 1e-5 of the f32 restatement, the distance to the
+    FP64 oracle the f32-vs-f64 floor, and no non-finite pixel (a path whose
+    throughput turns non-finite, 0 * inf over an underflowed sampling pdf, ends)."""
+    scene = W.microfacet_spheres()
+    settings = RenderSettings(total_samples=64, max_bounces=16, firefly_clamp=float("inf"))
+    desc = make_render_desc(32, 24, 64, 64, settings, seed=11, flags=EXT, rr_depth=4)
+    ref32, _, _ = oracle_lib.OracleScene(scene, "f32").render(desc)
+    ref64, _, _ = oracle_lib.OracleScene(scene, "f64").render(desc, prune=True)
+    announce("extended estimator, microfacet spheres 32x24x64")
+    img = np.zeros((24, 32, 4), np.float32)
+    scene.build_acceleration(gpu_ctx).render(img, desc)
+    assert np.isfinite(img).all() and np.isfinite(ref32).all()
+    rel = np.abs(img - ref32)[..., :3] / np.maximum(np.abs(ref32[..., :3]), 1e-3)
+    floor = relmse(ref32, ref64)
+    print(f"[extended] microfacet: max rel vs f32 {rel.max():.3e}, relMSE vs FP64 {relmse(img, ref64):.3e} "
+          f"(floor {floor:.3e})", flush=True)
+    assert rel.max() <= 1e-5
+    assert relmse(img, ref64) <= 1.5 * floor + 1e-7

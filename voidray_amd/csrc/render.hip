@@ -416,8 +416,11 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                             A = mulv(A, sc.att);
                             float pb = 0.f;
                             if (sc.diffuse) pb = fmaxf(0.f, dot3(sc.lobe, sc.dir)) / kPi;
-                            bool alive = true;
-                            if ((r.flags & VR_RENDER_RUSSIAN_ROULETTE) && depth >= r.rr_depth) {
+                            // a non-finite throughput (a zero BSDF value over a sampling pdf
+                            // that underflowed: 0 * inf) ends the path instead of reaching
+                            // the pixel as NaN; the reference estimator's clamp handles it
+                            bool alive = isfinite(A.x) && isfinite(A.y) && isfinite(A.z);
+                            if (alive && (r.flags & VR_RENDER_RUSSIAN_ROULETTE) && depth >= r.rr_depth) {
                                 float q = fminf(0.95f, fmaxf(A.x, fmaxf(A.y, A.z)));
                                 if (rng.uniform() >= q) alive = false;
                                 else A = A / q;
