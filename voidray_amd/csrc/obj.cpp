@@ -15,6 +15,7 @@
 // mesh.rs then keeps position, normal and (u, v) of each vertex.  The
 // reference unwraps every error (a panic); here the call returns
 // VR_ERR_INVALID with the file, line and reason.
+#include <algorithm>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
