@@ -13,7 +13,7 @@ constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 constexpr int kLdsStack = 10;
 constexpr int kLdsStackAny = 8;  // (registers, not LDS, bound the resident waves)
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
-constexpr int kRefill = 24;      // idle lanes that trigger a wave-wide refill (closest hit)
+constexpr int kRefill = 32;      // idle lanes that trigger a wave-wide refill (closest hit; 16, 24 and 40 measured slower)
 constexpr int kRefillAny = 40;   // the same for any-hit queries (shorter rays: refill less often)
 constexpr int kNodeSteps = 4;    // most node-phase iterations between two leaf phases
 
