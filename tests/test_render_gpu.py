@@ -73,6 +73,20 @@ def test_render_matches_oracle(case, gpu_ctx, oracle_lib):
     assert float(np.abs(ref[..., :3]).sum()) > 0
 
 
+def test_leaf_reference_layout_renders_identically(gpu_ctx):
+    """Leaves holding references into one record per triangle
+    (VR_BUILD_LEAF_REFS) render the same image as leaves holding the records."""
+    scene = W.material_spheres()
+    desc = make_render_desc(48, 48, 4, 4, RenderSettings(total_samples=4, max_bounces=10), seed=3)
+    announce("render with both leaf layouts")
+    imgs = []
+    for flags in (N.VR_BUILD_LEAF_RECORDS, N.VR_BUILD_LEAF_REFS):
+        img = np.zeros((48, 48, 4), np.float32)
+        scene.build_acceleration(gpu_ctx, build_flags=flags).render(img, desc)
+        imgs.append(img)
+    assert np.array_equal(imgs[0], imgs[1])
+
+
 def test_non_square_rows_by_height(gpu_ctx, oracle_lib):
     """iterative.rs:26 computes y = index / height; for width != height the row
     index wraps in u32 (iterative.rs:33).  Reproduced bit for bit."""
@@ -160,10 +174,13 @@ def test_tonemap_matches_oracle(mode, gpu_ctx, oracle_lib):
 
 
 def test_scene_from_files(tmp_path, gpu_ctx, oracle_lib):
-    """A mesh and a texture written to disk and read back the way the reference
-    reads its assets (Mesh::from_file with obj-rs, image::open(..).to_rgb32f()),
-    rendered on the GPU against the oracle."""
+    """A mesh, a texture and an HDR sky written to disk (OBJ, PNG, OpenEXR with
+    ZIP compression) and read back the way the reference reads its assets
+    (Mesh::from_file with obj-rs, image::open(..).to_rgb32f()), rendered on the
+    GPU against the oracle."""
     from PIL import Image
+
+    from tests.test_files import _exr_bytes
 
     from voidray_amd.scene import Camera, Environments, Materials, SampleType, Scene, Surfaces
     sph = W.uv_sphere((0.0, 0.5, 0.0), 0.5, 24, 12)
@@ -180,7 +197,10 @@ def test_scene_from_files(tmp_path, gpu_ctx, oracle_lib):
     sc.add_object(sc.add_material(Materials.lambertian_texture_no_normal(t)), surf)
     sc.add_object(sc.add_material(Materials.lambertian((0.5, 0.5, 0.5))), sc.add_analytic_surface(Surfaces.ground_plane(0.0)))
     sc.camera = Camera.look_at((0.0, 1.0, 2.5), (0.0, 0.4, 0.0), (0.0, 1.0, 0.0), 0.8)
-    sc.environment = Environments.hdri(W.procedural_sky(64, 32, sun_peak=50.0, sun_sharpness=40.0))
+    sky = W.procedural_sky(64, 32, sun_peak=50.0, sun_sharpness=40.0)
+    (tmp_path / "sky.exr").write_bytes(_exr_bytes({c: sky[..., k] for k, c in enumerate("RGB")}, 3, 2))
+    sc.environment = Environments.hdri(str(tmp_path / "sky.exr"))
+    assert np.array_equal(sc.environment.image, sky)
     assert sc.meshes[0].num_triangles == len(sph.indices)
     announce("render scene read from OBJ + PNG files")
     img, ref, stats, ost = _render_pair(sc, gpu_ctx, oracle_lib, 48, 40, 4, RenderSettings(total_samples=4, max_bounces=8))
