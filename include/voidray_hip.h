@@ -173,7 +173,7 @@ typedef struct vr_scene_stats {
     uint32_t num_ref_nodes;     /* flattened reference median-split nodes */
     uint32_t qbvh_depth;
     uint32_t max_leaf_size;
-    double build_ref_seconds;   /* median-split trees (ranks + fallback) */
+    double build_ref_seconds;   /* median-split trees (winner confirmation + exact engine) */
     double build_sah_seconds;   /* SAH build + collapse + quantisation   */
     double upload_seconds;
     uint64_t device_bytes;

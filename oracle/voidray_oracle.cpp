@@ -356,9 +356,13 @@ struct Counters {
 };
 
 // BvhNode::hit + merge_optionals (bvh.rs:132-181) for a mesh tree: both
-// children, unshrunk t_max; equal t keeps the right (later) result, which an
-// in-order walk with `t <= best` reproduces.  With prune = true the walk skips
-// boxes beyond the current best (used only for the FP64 convergence oracle).
+// children, unshrunk t_max.  A leaf yields (HitRecord, handle) (bvh.rs:143-147)
+// and merge_optionals keeps `left` iff left < right under Rust's lexicographic
+// tuple order, where HitRecord compares by t alone (ray.rs:52-62): on equal t
+// the smaller handle (triangle index) wins.  The merge is a minimum over a
+// total order, so a running minimum over the reached leaves in any visiting
+// order gives the recursion's answer.  With prune = true the walk skips boxes
+// beyond the current best (used only for the FP64 convergence oracle).
 template <class R>
 bool mesh_tree_hit(const Mesh<R>& m, const Ray<R>& ray, R tmin, Hit<R>& best, bool have, bool prune, Counters& ctr) {
     int32_t stack[128];
@@ -370,7 +374,7 @@ bool mesh_tree_hit(const Mesh<R>& m, const Ray<R>& ray, R tmin, Hit<R>& best, bo
             Hit<R> h;
             ++ctr.tris;
             if (tri_hit(m, (uint32_t)nd.obj, ray, tmin, h)) {
-                if (!have || h.t <= best.t) {
+                if (!have || h.t < best.t || (h.t == best.t && h.tri < best.tri)) {
                     best = h;
                     have = true;
                 }
@@ -447,7 +451,9 @@ template <class R> bool analytic_hit(const vr_analytic& a, const Ray<R>& ray, R 
     return true;
 }
 
-// SceneAcceleration::hit (scene.rs:182-185) via the top-level tree.
+// SceneAcceleration::hit (scene.rs:182-185) via the top-level tree: the same
+// merge over (HitRecord, surface index) tuples, so on equal t the smaller
+// surface index wins (the mesh's own tie was settled inside mesh_hit).
 template <class R> bool scene_hit(const Scene<R>& s, const Ray<R>& ray, R tmin, Hit<R>& best, bool prune, Counters& ctr) {
     if (s.top_root < 0) return false;
     bool have = false;
@@ -461,7 +467,7 @@ template <class R> bool scene_hit(const Scene<R>& s, const Ray<R>& ray, R tmin, 
             Hit<R> h;
             if (s.is_analytic(surf) ? analytic_hit(s.analytic_of(surf), ray, tmin, h)
                                     : mesh_hit(s.meshes[s.surface_mesh[surf]], ray, tmin, h, prune, ctr)) {
-                if (!have || h.t <= best.t) {
+                if (!have || h.t < best.t || (h.t == best.t && surf < best.surface)) {
                     h.surface = surf;
                     best = h;
                     have = true;

@@ -158,35 +158,33 @@ def test_median_split_invariants():
     assert sorted(orc.leaf_order(0)) == list(range(1000))
 
 
-def test_traversal_equals_brute_force_with_rank_ties():
+def test_traversal_equals_brute_force_with_handle_ties():
+    # the oracle's brute force resolves equal t by the rank it is given:
+    # rank = -handle is the reference's rule (smallest triangle wins)
     scene = W.random_triangle_scene(3000, seed=23)
     orc = oracle.OracleScene(scene)
-    order = orc.leaf_order(0)
-    rank = np.zeros(len(order), np.uint32)
-    rank[order] = np.arange(len(order))
+    rank = np.arange(3000, dtype=np.uint32)[::-1].copy()
     o, d = W.microbench_rays(4000, res=64)
     a = orc.trace(o, d)
     b = orc.trace_bruteforce(o, d, rank)
     assert np.array_equal(a["prim"], b["prim"]) and np.array_equal(a["t"], b["t"])
 
 
-def test_equal_t_tie_keeps_later_leaf():
-    # two coincident triangles: merge_optionals keeps the right (later in-order) leaf
-    # (bvh.rs:171-175); the brute-force small-mesh loop keeps the earlier (mesh.rs:131)
+def test_equal_t_tie_keeps_smallest_triangle():
+    # six coincident triangles: every leaf returns the same t, and
+    # merge_optionals compares (HitRecord, handle) tuples whose HitRecord part
+    # orders by t alone (bvh.rs:164-181, ray.rs:52-62), so the smallest
+    # triangle index wins.  The brute-force small-mesh loop keeps the earlier
+    # triangle with its strict `>` (mesh.rs:131): the same answer.
     tri = [[0, 0, 1], [1, 0, 1], [0, 1, 1]]
-    pos = np.array(tri * 6, f32)
-    idx = np.arange(18, dtype=np.uint32).reshape(6, 3)  # 6 copies: mesh has a BVH (> 4 tris)
-    sc = Scene.empty()
-    sc.add_object(sc.add_material(Materials.lambertian((1, 1, 1))), sc.add_mesh(Mesh(pos, idx)))
-    orc = oracle.OracleScene(sc)
-    r = orc.trace(np.array([[0.2, 0.2, 0]], f32), np.array([[0, 0, 1]], f32))
-    assert r["prim"][0] == orc.leaf_order(0)[-1]
-    pos = np.array(tri * 3, f32)
-    idx = np.arange(9, dtype=np.uint32).reshape(3, 3)  # 3 copies: brute force
-    sc = Scene.empty()
-    sc.add_object(sc.add_material(Materials.lambertian((1, 1, 1))), sc.add_mesh(Mesh(pos, idx)))
-    r = oracle.OracleScene(sc).trace(np.array([[0.2, 0.2, 0]], f32), np.array([[0, 0, 1]], f32))
-    assert r["prim"][0] == 0
+    for copies in (6, 3):  # 6: the mesh has a tree (> 4 tris); 3: brute force
+        pos = np.array(tri * copies, f32)
+        idx = np.arange(3 * copies, dtype=np.uint32).reshape(copies, 3)
+        sc = Scene.empty()
+        sc.add_object(sc.add_material(Materials.lambertian((1, 1, 1))), sc.add_mesh(Mesh(pos, idx)))
+        orc = oracle.OracleScene(sc)
+        r = orc.trace(np.array([[0.2, 0.2, 0]], f32), np.array([[0, 0, 1]], f32))
+        assert r["prim"][0] == 0 and r["t"][0] == 1.0, copies
 
 
 def test_flat_split_box_hides_surfaces():
@@ -420,15 +418,15 @@ def test_ground_plane_closed_form():
     assert abs(r["t"][3] - 1.5 / s) < 1e-5 and abs(r["u"][3] - 1.5) < 1e-5
 
 
-def test_coincident_spheres_tie_keeps_later_surface():
-    # two identical spheres: equal t; equal centroids keep insertion order in
-    # the median split, so the later surface is the right leaf and wins
-    # (merge_optionals, bvh.rs:164-181)
+def test_coincident_spheres_tie_keeps_smallest_surface():
+    # two identical spheres give equal t; the top-level merge compares
+    # (HitRecord, surface index) tuples (scene.rs:182-185, bvh.rs:164-181), so
+    # surface 0 wins whichever leaf comes later in the tree
     from voidray_amd.scene import Surfaces
     orc = oracle.OracleScene(_analytic_scene(Surfaces.sphere((0.0, 0.0, 0.0), 1.0),
                                              Surfaces.sphere((0.0, 0.0, 0.0), 1.0)))
     r = orc.trace(np.float32([[0, 5, 0], [0.3, 0.2, 4]]), np.float32([[0, -1, 0], [0, 0, -1]]))
-    assert r["prim"].tolist() == [1, 1] and r["t"][0] == 4.0
+    assert r["prim"].tolist() == [0, 0] and r["t"][0] == 4.0
 
 
 def test_microfacet_bsdf_closed_form():

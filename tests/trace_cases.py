@@ -45,3 +45,94 @@ def compare_hits(ref, got, label):
     for k in ("t", "u", "v"):
         diff = np.nonzero(np.asarray(ref[k], np.float32)[h].view(np.uint32) != got[k][h].view(np.uint32))[0]
         assert len(diff) == 0, f"{label}: {len(diff)} {k} mismatches"
+
+
+def _dot(a, b):
+    # (x*x' + y*y') + z*z', every operation rounded to f32 (numpy float32 ufuncs)
+    return (a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1]) + a[..., 2] * b[..., 2]
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1)
+
+
+def mt_numpy(o, d, p0, p1, p2, tmin=np.float32(1e-5)):
+    """Triangle::hit (voidray_renderer/src/core/mesh.rs:144-188) for every
+    (ray, triangle) pair, written independently of the oracle: o, d (R, 3),
+    p0..p2 (T, 3), all float32.  Returns (hit (R, T) bool, t (R, T))."""
+    o = o[:, None, :]
+    d = d[:, None, :]
+    e1 = (p1 - p0)[None]
+    e2 = (p2 - p0)[None]
+    h = _cross(d, e2)
+    a = _dot(e1, h)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ok = ~((a > -tmin) & (a < tmin))
+        f = np.float32(1.0) / a
+        s = o - p0[None]
+        u = f * _dot(s, h)
+        ok &= ~((u < 0) | (u > 1))
+        q = _cross(s, e1)
+        v = f * _dot(d, q)
+        ok &= ~((v < 0) | (u + v > 1))
+        t = f * _dot(e2, q)
+        ok &= t > tmin
+    return ok, t
+
+
+def sphere_numpy(o, d, c, r, tmin=np.float32(1e-5)):
+    """Sphere::hit (voidray_common/src/surfaces.rs:46-78) with t_max = inf, f32."""
+    c = np.asarray(c, np.float32)
+    r = np.float32(r)
+    oc = o - c
+    aa = _dot(d, d)
+    half_b = _dot(oc, d)
+    cc = _dot(oc, oc) - r * r
+    disc = half_b * half_b - aa * cc
+    with np.errstate(invalid="ignore"):
+        sq = np.sqrt(disc)
+        root = (-half_b - sq) / aa
+        far = (root < tmin) | (np.float32(np.inf) < root)
+        root = np.where(far, (-half_b + sq) / aa, root)
+        ok = (disc >= 0) & ~((root < tmin) | (np.float32(np.inf) < root))
+    return ok, root
+
+
+def bruteforce_lowest_handle(scene, o, d):
+    """Closest hit over every primitive with no boxes, ties to the smallest
+    (surface, triangle) handle pair: the order merge_optionals' tuple compare
+    gives (bvh.rs:164-181 with ray.rs:52-62) and the brute-force mesh loop's
+    strict `>` (mesh.rs:131).  Returns prim (global id, surface by surface),
+    t, and the surface index of the winner."""
+    from voidray_amd import _native as N
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    n = len(o)
+    best_t = np.full(n, np.inf, np.float32)
+    best_p = np.full(n, 0xFFFFFFFF, np.uint32)
+    best_s = np.full(n, -1, np.int64)
+    base = 0
+    for si, sm in enumerate(scene.surfaces):
+        if sm & N.VR_SURFACE_ANALYTIC:
+            a = scene.analytic[sm & ~N.VR_SURFACE_ANALYTIC]
+            assert a.kind == N.VR_ANALYTIC_SPHERE, "brute force covers spheres and meshes"
+            ok, t = sphere_numpy(o, d, a.center, a.radius)
+            hit = ok[:, None]
+            tt = t[:, None]
+            ids = np.array([base], np.uint32)
+        else:
+            m = scene.meshes[sm]
+            P = m.positions[m.indices]
+            hit, tt = mt_numpy(o, d, P[:, 0], P[:, 1], P[:, 2])
+            ids = base + np.arange(len(m.indices), dtype=np.uint32)
+        # candidates in handle order: a strict `<` keeps the first of equal t
+        for k in range(hit.shape[1]):
+            take = hit[:, k] & (tt[:, k] < best_t)
+            best_t = np.where(take, tt[:, k], best_t)
+            best_p = np.where(take, ids[k], best_p)
+            best_s = np.where(take, si, best_s)
+        base += len(ids)
+    best_t = np.where(best_p == 0xFFFFFFFF, np.float32(0), best_t)
+    return dict(prim=best_p, t=best_t, surface=best_s)

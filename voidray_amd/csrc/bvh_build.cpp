@@ -1,11 +1,11 @@
 // Host-side acceleration-structure builders.
 //
 //  * build_reference_tree: the median-split binary tree of
-//    voidray_renderer/src/core/bvh.rs:48-130, flattened in pre-order.  It is
-//    needed for two things: the in-order leaf rank that resolves equal-t ties
-//    exactly as merge_optionals (bvh.rs:164-181) does, and the exact
-//    reference engine that re-traces the rare rays whose closest hit cannot be
-//    confirmed by the fast engine.
+//    voidray_renderer/src/core/bvh.rs:48-130, flattened in pre-order.  Its
+//    boxes decide which triangles the reference reaches: the fast engine
+//    confirms its winner against the ancestors' slab tests, and the exact
+//    reference engine walks it for the rare rays whose closest hit cannot be
+//    confirmed that way.
 //  * build_qbvh: a binned-SAH binary BVH with spatial splits collapsed to
 //    4-wide nodes with 8-bit outward-rounded child boxes (QNode, 64 B) and
 //    triangle records in leaf order.
@@ -119,10 +119,9 @@ struct RefBuilder {
 
 // Rebase the `right` links of a subtree that was built into its own vector
 // starting at index 0 (done by the parallel branch above).
-void build_reference_tree(const std::vector<Box3>& bounds, std::vector<RefNode>& out, std::vector<uint32_t>& rank) {
+void build_reference_tree(const std::vector<Box3>& bounds, std::vector<RefNode>& out) {
     out.clear();
     size_t n = bounds.size();
-    rank.assign(n, 0);
     if (n == 0) return;
     RefBuilder b(bounds);
     std::vector<uint32_t> objs(n);
@@ -132,10 +131,6 @@ void build_reference_tree(const std::vector<Box3>& bounds, std::vector<RefNode>&
     int par = 0;
     while ((1u << par) < hw && par < 6) ++par;
     b.build(objs, out, par);
-    // in-order leaf rank (pre-order layout: leaves appear in in-order order)
-    uint32_t r = 0;
-    for (auto& nd : out)
-        if (nd.obj >= 0) rank[nd.obj] = r++;
 }
 
 // ---------------------------------------------------------------------------
@@ -270,7 +265,7 @@ static bool clipped_box(const TriRec& t, const BBox& box, BBox& out) {
 // Every point of a triangle lies in the piece on its side of each cut, and
 // each piece box is rounded outward and padded, so a ray that hits the
 // triangle reaches at least one leaf holding it; the copies of a triangle give
-// the same t and rank, so the closest hit does not change.
+// the same t and prim id, so the closest hit does not change.
 struct SRef {
     BBox box;
     uint32_t tri;

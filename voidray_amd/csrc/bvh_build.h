@@ -5,9 +5,8 @@
 #include "internal.h"
 
 namespace vr {
-// Median-split tree of bvh.rs:48-130 over `bounds`, flattened in pre-order;
-// rank[i] = in-order leaf position of object i.
-void build_reference_tree(const std::vector<Box3>& bounds, std::vector<RefNode>& out, std::vector<uint32_t>& rank);
+// Median-split tree of bvh.rs:48-130 over `bounds`, flattened in pre-order.
+void build_reference_tree(const std::vector<Box3>& bounds, std::vector<RefNode>& out);
 // SAH-built 4-wide quantised BVH; tris_out = tris_in in leaf order.
 int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, std::vector<TriRec>& tris_out,
                uint32_t& depth, std::string& err);

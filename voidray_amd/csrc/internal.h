@@ -39,11 +39,14 @@ constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
 constexpr uint32_t kMaxLeaf = 4;  // triangles per leaf (count-1 fits 3 bits)
 
 // Triangle record in leaf order (48 B): world-space vertices exactly as the
-// reference stores them, the global prim id and the reference tie rank.
+// reference stores them and the global prim id.  Prim ids number the
+// triangles surface by surface (surface index, then triangle index within the
+// mesh), so the smaller prim id is the smaller (surface, triangle) handle pair
+// that merge_optionals keeps on equal t (bvh.rs:164-181, ray.rs:52-62).
 struct alignas(16) TriRec {
     float p0[3], p1[3], p2[3];
     uint32_t prim;
-    uint32_t rank;
+    uint32_t reserved;
     uint32_t flags;  // bit 0: the owning mesh has a reference BVH (box tests apply); bits 1..31: surface
 };
 static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
@@ -68,7 +71,6 @@ struct MeshInfo {
 struct SurfaceInfo {
     uint32_t mesh;         // mesh index (analytic surfaces: kNoMesh)
     uint32_t prim_base;    // global prim id of the surface's first triangle
-    uint32_t rank_base;    // tie-rank offset (top-level in-order position)
     uint32_t material;     // objects[surface].material (scene.rs:184)
     uint32_t path_off;     // top-level ancestors: indices into top_path
     uint32_t path_len;
@@ -77,12 +79,12 @@ struct SurfaceInfo {
 constexpr uint32_t kNoMesh = 0xFFFFFFFFu;
 
 // Analytic surface (voidray_common/src/surfaces.rs:31-114) with its surface
-// index, tie rank and prim id.
+// index and prim id.
 struct alignas(16) AnalyticDev {
     int32_t kind;          // VR_ANALYTIC_*
     float c[3];            // sphere centre
     float r, h;            // sphere radius, ground-plane height
-    uint32_t surface, rank, prim;
+    uint32_t surface, prim;
 };
 
 struct GpuMaterial {
@@ -112,7 +114,6 @@ struct SceneView {
     const uint32_t* top_path;     // per-surface ancestor lists (top_nodes indices)
     const MeshInfo* meshes;
     const SurfaceInfo* surfaces;
-    const uint32_t* mesh_rank;    // per mesh triangle (concatenated): in-order rank within the mesh
     const uint32_t* mesh_idx;     // per mesh triangle: 3 vertex indices (mesh-local)
     const float* face_n;          // per mesh triangle: face normal (mesh.rs:80-84)
     const float* vpos;            // concatenated vertices
@@ -185,7 +186,7 @@ struct HostScene {
     std::vector<TriRec> tris;
     std::vector<uint32_t> tri_ref;  // leaf order -> tris index; empty: tris are in leaf order
     std::vector<RefNode> ref_nodes, top_nodes;
-    std::vector<uint32_t> top_path, mesh_rank, mesh_idx, prim_surface;
+    std::vector<uint32_t> top_path, mesh_idx, prim_surface;
     std::vector<int32_t> ref_parent, prim_anc, mesh_leaf_parent;
     std::vector<MeshInfo> meshes;
     std::vector<SurfaceInfo> surfaces;

@@ -4,11 +4,11 @@
 //
 // HBM layout (all arrays 256-byte aligned, one allocation each):
 //   qnodes   QNode[num_qnodes]      64 B  SAH 4-wide with spatial splits, 8-bit outward-quantised
-//   tris     TriRec[R]              48 B  world vertices in leaf order + prim + rank (R >= T:
+//   tris     TriRec[R]              48 B  world vertices in leaf order + prim id (R >= T:
 //                                         one record per leaf holding a piece of a split triangle)
 //   ref/top  RefNode[]              32 B  reference median-split trees (exact engine)
 //   ref_parent, prim_anc            4 B   reference-tree parents, per prim its leaf's parent
-//   per mesh triangle: idx (12 B), face normal (12 B), in-order rank (4 B)
+//   per mesh triangle: idx (12 B), face normal (12 B)
 //   per vertex: position, normal (12 B each), uv (8 B)
 //   materials, textures (RGBA8 or RGB f32 texels), HDRI (RGB f32) + IS tables
 #include <algorithm>
@@ -78,7 +78,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     std::vector<MeshInfo>& minfo = H.meshes;
     minfo.assign(M, MeshInfo{});
     std::vector<float>&vpos = H.vpos, &vnrm = H.vnrm, &vuv = H.vuv, &face_n = H.face_n;
-    std::vector<uint32_t>&midx = H.mesh_idx, &mrank = H.mesh_rank;
+    std::vector<uint32_t>& midx = H.mesh_idx;
     std::vector<RefNode>& ref_nodes = H.ref_nodes;
     std::vector<int32_t>&ref_parent = H.ref_parent, &leaf_parent = H.mesh_leaf_parent;
     std::vector<Box3> mesh_bounds(M);
@@ -93,7 +93,6 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     vuv.reserve(2 * tv);
     midx.reserve(3 * tt);
     face_n.reserve(3 * tt);
-    mrank.reserve(tt);
     for (uint32_t m = 0; m < M; ++m) {
         const vr_mesh& src = d->meshes[m];
         if (src.num_vertices && !src.positions) return fail(VR_ERR_INVALID, "mesh without positions");
@@ -134,8 +133,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
         }
         if (src.num_triangles > 4) {  // SMALL_MESH (mesh.rs:43, 111-114)
             std::vector<RefNode> tree;
-            std::vector<uint32_t> rank;
-            build_reference_tree(tb, tree, rank);
+            build_reference_tree(tb, tree);
             int32_t off = (int32_t)ref_nodes.size();
             for (auto& nd : tree)
                 if (nd.obj < 0) nd.right += off;
@@ -155,10 +153,6 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             }
             for (size_t k = 0; k < tree.size(); ++k)
                 if (tree[k].obj >= 0) leaf_parent[mi.tri_base + (uint32_t)tree[k].obj] = ref_parent[off + k];
-            mrank.insert(mrank.end(), rank.begin(), rank.end());
-        } else {
-            // brute-force loop keeps the earlier triangle on equal t (mesh.rs:131)
-            for (uint32_t t = 0; t < src.num_triangles; ++t) mrank.push_back(src.num_triangles - 1 - t);
         }
     }
     ref_parent.resize(ref_nodes.size(), -1);
@@ -185,18 +179,16 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
         }
     }
     std::vector<RefNode>& top = H.top_nodes;
-    std::vector<uint32_t> top_rank;
-    build_reference_tree(sb, top, top_rank);
+    build_reference_tree(sb, top);
     std::vector<SurfaceInfo>& sinfo = H.surfaces;
     sinfo.assign(S, SurfaceInfo{});
-    std::vector<uint32_t> order(S);
-    for (uint32_t i = 0; i < S; ++i) order[top_rank[i]] = i;
-    uint32_t rb = 0;
-    for (uint32_t k = 0; k < S; ++k) {
-        uint32_t sidx = order[k];
-        sinfo[sidx].rank_base = rb;
-        rb += surface_prims(sidx);
-    }
+    // Global prim ids run surface by surface, and within a mesh in triangle
+    // order: on equal t the reference keeps the smaller handle (merge_optionals
+    // compares (HitRecord, handle) tuples whose HitRecord part orders by t
+    // alone, bvh.rs:143-175 and ray.rs:52-62; the brute-force mesh loop keeps
+    // the earlier triangle, mesh.rs:131), first among surfaces (scene.rs:183),
+    // then among the triangles of the winning mesh, so the winner of a tie is
+    // the smallest prim id.
     uint32_t pb = 0;
     std::vector<AnalyticDev>& analytic = H.analytic;
     for (uint32_t i = 0; i < S; ++i) {
@@ -213,7 +205,6 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             a.r = an.radius;
             a.h = an.height;
             a.surface = i;
-            a.rank = sinfo[i].rank_base;
             a.prim = pb;
             analytic.push_back(a);
         }
@@ -270,7 +261,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
                 r.p2[a] = vpos[3 * (size_t)(mi.vert_base + ix[2]) + a];
             }
             r.prim = sinfo[i].prim_base + t;
-            r.rank = sinfo[i].rank_base + mrank[mi.tri_base + t];
+            r.reserved = 0;
             r.flags = (mi.root >= 0 ? 1u : 0u) | (i << 1);
             prim_surface[sinfo[i].prim_base + t] = i;
             prim_anc[sinfo[i].prim_base + t] = leaf_parent[mi.tri_base + t];
@@ -379,7 +370,6 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.top_path = top_path.data();
     v.meshes = minfo.data();
     v.surfaces = sinfo.data();
-    v.mesh_rank = mrank.data();
     v.mesh_idx = midx.data();
     v.face_n = face_n.data();
     v.vpos = vpos.data();
@@ -433,7 +423,6 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.top_path, &v.top_path);
     rc |= upload(s, H.meshes, &v.meshes);
     rc |= upload(s, H.surfaces, &v.surfaces);
-    rc |= upload(s, H.mesh_rank, &v.mesh_rank);
     rc |= upload(s, H.mesh_idx, &v.mesh_idx);
     rc |= upload(s, H.face_n, &v.face_n);
     rc |= upload(s, H.vpos, &v.vpos);

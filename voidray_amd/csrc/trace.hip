@@ -208,7 +208,7 @@ __device__ __forceinline__ void exact_serial(const TraceArgs& a, uint32_t i, int
 // candidate set does not depend on visiting order; the wave pops up to 64
 // pending nodes at once from an LDS stack, tests them in parallel and pushes
 // the children of passing nodes.  The winner is the lexicographic minimum of
-// (t, -rank) over the wave, identical to trace_exact.  A ray whose frontier
+// (t, prim) over the wave, identical to trace_exact.  A ray whose frontier
 // outgrows the LDS stack is listed for the serial engine, which the last block
 // to finish runs over the list, one ray per lane.  `list` == nullptr: every
 // ray (or every queued ray) is traced; otherwise the wide engine's fallbacks.
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
         vf3 o = mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]);
         vf3 d = mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]);
         float bt = kInf, bu = 0.f, bv = 0.f;
-        uint32_t brank = 0, bprim = 0xFFFFFFFFu;
+        uint32_t bprim = 0xFFFFFFFFu;
         int sp = sv.top_root_valid ? 1 : 0;  // wave-uniform
         if (lane == 0) stk[0] = u32x2{kCoopTop << 30, 0u};
         bool overflow = false;
@@ -265,9 +265,8 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         if (si2.analytic >= 0) {  // analytic surface: its hit, in this lane
                             float t, u, v;
                             if (analytic_hit(sv.analytic[si2.analytic], o, d, a.tmin, t, u, v) &&
-                                (t < bt || (t == bt && si2.rank_base > brank))) {
+                                (t < bt || (t == bt && si2.prim_base < bprim))) {
                                 bt = t;
-                                brank = si2.rank_base;
                                 bu = u;
                                 bv = v;
                                 bprim = si2.prim_base;
@@ -318,13 +317,12 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
                         MTResult r;
                         if (mt_hit(o, d, p0, p1, p2, a.tmin, r)) {
-                            uint32_t rank = si.rank_base + sv.mesh_rank[mi.tri_base + (uint32_t)tri];
-                            if (r.t < bt || (r.t == bt && rank > brank)) {
+                            const uint32_t prim = si.prim_base + (uint32_t)tri;
+                            if (r.t < bt || (r.t == bt && prim < bprim)) {
                                 bt = r.t;
-                                brank = rank;
                                 bu = r.u;
                                 bv = r.v;
-                                bprim = si.prim_base + (uint32_t)tri;
+                                bprim = prim;
                             }
                         }
                     }
@@ -356,18 +354,16 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
             }
             continue;
         }
-        // wave argmin of (t, -rank)
+        // wave argmin of (t, prim)
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             float ot = __shfl_xor(bt, off);
-            uint32_t orank = __shfl_xor(brank, off);
             float ou = __shfl_xor(bu, off), ov = __shfl_xor(bv, off);
             uint32_t oprim = __shfl_xor(bprim, off);
             bool take_other = oprim != 0xFFFFFFFFu &&
-                              (bprim == 0xFFFFFFFFu || ot < bt || (ot == bt && orank > brank));
+                              (bprim == 0xFFFFFFFFu || ot < bt || (ot == bt && oprim < bprim));
             if (take_other) {
                 bt = ot;
-                brank = orank;
                 bu = ou;
                 bv = ov;
                 bprim = oprim;

@@ -8,7 +8,7 @@
 //            shrinking.  Box tests are conservative (padded boxes, a relative
 //            slack on the slab exit, a relative slack on the cull distance), so
 //            every triangle the reference could return is reached.  Equal-t
-//            hits are resolved by the reference's in-order rank.  The winner is
+//            hits go to the smaller prim id (see below).  The winner is
 //            then confirmed to be a triangle the reference reaches: its
 //            top-level ancestors must pass the exact reference slab test, and
 //            so must its mesh-tree ancestors — taken as passed when the hit
@@ -20,8 +20,15 @@
 //            (core/bvh.rs:132-181): both children, unshrunk t_max, no box test
 //            at leaves, brute force for meshes of <= 4 triangles
 //            (core/mesh.rs:125-138), with the same slab arithmetic
-//            (util/aabb.rs:86-148).  Equal t is resolved by rank, which equals
-//            merge_optionals' right-wins rule and the brute-force first-wins rule.
+//            (util/aabb.rs:86-148).
+//
+// Ties.  merge_optionals (bvh.rs:164-181) compares Option<(HitRecord, usize)>
+// tuples, and HitRecord orders by t alone (ray.rs:52-62), so on equal t the
+// smaller handle wins: the smaller surface index at the top level
+// (scene.rs:182-185), then the smaller triangle index inside the mesh
+// (mesh.rs:269-279; the brute-force loop of mesh.rs:125-138 keeps the earlier
+// triangle, the same rule).  Prim ids are numbered surface by surface in
+// triangle order, so both engines keep the lexicographic minimum of (t, prim).
 #pragma once
 #include "internal.h"
 
@@ -38,8 +45,8 @@ constexpr float kRobustRel = 1.0f / 1048576.0f;        // 2^-20 interior margin
 
 
 // Geometry of a triangle record: two 16-byte loads + one 4-byte load, all
-// float.  The integer words (prim = 9, rank = 10, flags = 11) are read with
-// their own scalar loads, only when a hit needs them.
+// float.  The integer words (prim = 9, flags = 11) are read with their own
+// loads, only when a hit needs them.
 VR_HD void load_tri_geom(const TriRec* tris, uint32_t slot, vf3& p0, vf3& p1, vf3& p2) {
     const f32x4* q = (const f32x4*)(tris + slot);
     f32x4 a = q[0], b = q[1];
@@ -143,20 +150,20 @@ VR_HD bool top_path_pass(const SceneView& sv, uint32_t surf, vf3 o, vf3 d, float
 }
 
 // Closest analytic surface the reference reaches (lexicographic min of (t,
-// -rank), like the triangles); with `first_below` set, the first one found
+// prim), like the triangles); with `first_below` set, the first one found
 // with t < first_below.  Returns its index or -1.
-VR_HD int analytic_best(const SceneView& sv, vf3 o, vf3 d, float tmin, float first_below, float& bt, uint32_t& brank,
+VR_HD int analytic_best(const SceneView& sv, vf3 o, vf3 d, float tmin, float first_below, float& bt, uint32_t& bprim,
                         float& bu, float& bv) {
     int best = -1;
     for (uint32_t k = 0; k < sv.num_analytic; ++k) {
         const AnalyticDev& a = sv.analytic[k];
         float t, u, v;
         if (!analytic_hit(a, o, d, tmin, t, u, v)) continue;
-        if (best >= 0 && !(t < bt || (t == bt && a.rank > brank))) continue;
+        if (best >= 0 && !(t < bt || (t == bt && a.prim < bprim))) continue;
         if (!top_path_pass(sv, a.surface, o, d, tmin)) continue;
         best = (int)k;
         bt = t;
-        brank = a.rank;
+        bprim = a.prim;
         bu = u;
         bv = v;
         if (t < first_below) break;
@@ -176,7 +183,7 @@ struct WideState {
     vf3 o, d;
     float idx, idy, idz;
     float tbest, bu, bv;
-    uint32_t best_rank, best_slot;
+    uint32_t best_prim, best_slot;
     uint32_t cur;
     int sp, lo;
     bool have, unconfirmed, found;
@@ -200,7 +207,7 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     st.idz = 1.0f / (fabsf(d.z) > kTiny ? d.z : copysignf(kTiny, d.z));
     st.tbest = kInf;
     st.bu = st.bv = 0.f;
-    st.best_rank = 0;
+    st.best_prim = 0xFFFFFFFFu;
     st.best_slot = 0xFFFFFFFFu;
     st.cur = 0;
     st.sp = 0;
@@ -212,13 +219,13 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
 
 // Analytic surfaces are few and not in the wide BVH: each ray tests them all
 // up front, and the best one seeds the closest hit (so the triangle walk culls
-// against it and ties resolve by rank as usual).  An any-hit ray that hits one
-// below tmax_any is finished.
+// against it and ties resolve by prim id as usual).  An any-hit ray that hits
+// one below tmax_any is finished.
 template <bool ANY>
 VR_HD void wide_seed_analytic(const SceneView& sv, WideState& st, float tmin, float tmax_any) {
     float t = kInf, u = 0.f, v = 0.f;
-    uint32_t rank = 0;
-    const int k = analytic_best(sv, st.o, st.d, tmin, ANY ? tmax_any : -kInf, t, rank, u, v);
+    uint32_t prim = 0xFFFFFFFFu;
+    const int k = analytic_best(sv, st.o, st.d, tmin, ANY ? tmax_any : -kInf, t, prim, u, v);
     if (k < 0) return;
     if (ANY) {
         if (!(t < tmax_any)) return;
@@ -226,7 +233,7 @@ VR_HD void wide_seed_analytic(const SceneView& sv, WideState& st, float tmin, fl
         st.have = false;
     }
     st.tbest = t;
-    st.best_rank = rank;
+    st.best_prim = prim;
     st.best_slot = kAnalyticSlot | (uint32_t)k;
     st.bu = u;
     st.bv = v;
@@ -304,10 +311,10 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
             continue;
         }
         if (r.t > st.tbest) continue;
-        uint32_t rank = tri_word(sv.tris, slot, 10);
-        if (r.t < st.tbest || rank > st.best_rank) {
+        const uint32_t prim = tri_word(sv.tris, slot, 9);
+        if (r.t < st.tbest || prim < st.best_prim) {
             st.tbest = r.t;
-            st.best_rank = rank;
+            st.best_prim = prim;
             st.best_slot = slot;
             st.bu = r.u;
             st.bv = r.v;
@@ -462,7 +469,6 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
     if (!sv.top_root_valid) return;
     bool have = false;
     float bt = 0.f;
-    uint32_t brank = 0;
     int tsp = 0;
     tstack[tsp++ * stride] = 0;
     while (tsp) {
@@ -481,10 +487,9 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
         if (si.analytic >= 0) {
             float t, u, v;
             if (analytic_hit(sv.analytic[si.analytic], o, d, tmin, t, u, v) &&
-                (!have || t < bt || (t == bt && si.rank_base > brank))) {
+                (!have || t < bt || (t == bt && si.prim_base < out.prim))) {
                 have = true;
                 bt = t;
-                brank = si.rank_base;
                 out.t = t;
                 out.u = u;
                 out.v = v;
@@ -501,15 +506,14 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
             vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
             MTResult r;
             if (!mt_hit(o, d, p0, p1, p2, tmin, r)) return;
-            uint32_t rank = si.rank_base + sv.mesh_rank[mi.tri_base + tri];
-            if (!have || r.t < bt || (r.t == bt && rank > brank)) {
+            const uint32_t prim = si.prim_base + tri;
+            if (!have || r.t < bt || (r.t == bt && prim < out.prim)) {
                 have = true;
                 bt = r.t;
-                brank = rank;
                 out.t = r.t;
                 out.u = r.u;
                 out.v = r.v;
-                out.prim = si.prim_base + tri;
+                out.prim = prim;
             }
         };
         if (mi.root < 0) {
