@@ -9,9 +9,17 @@ Pinning status (see tests/test_oracle.py):
   * slab test / Möller–Trumbore      — closed-form hits, edge and epsilon cases, brute force
   * median-split tree                — invariants + an independent recursive Python build
   * textures / HDRI lookups          — hand-built images with known bilinear answers
-  * materials (Schlick, reflect, refract, Marsaglia sphere) — closed forms, moments
+  * equal-t ties                     — an independent numpy brute force (tests/test_ties.py)
+  * materials (Schlick, reflect, refract, Metal, Lambertian, LambertianBSDF,
+    Dielectric, Emission), cast_ray (pinhole and thin lens), pixel -> NDC,
+    slab test, UnitDisc, environment importance sampling
+                                     — closed forms, Fresnel / Snell, symmetry,
+                                       normalisation by quadrature, chi-square
+                                       (tests/test_oracle_pins.py)
+  * Marsaglia sphere                 — moments and chi-square
   * integrator (nested clamp)        — special cases: emissive-only min(E, clamp),
                                        uniform environment with no geometry
+  * full-size microbench answers     — committed digests (tests/golden)
   * tonemaps                         — FP64 evaluations of the published curves
   * cgmath association order, rand/rand_distr streams, libm transcendentals:
     parity unpinned (third-party code not available here).
@@ -62,6 +70,21 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_tex_sample_f32.argtypes = [_P, C.c_float, C.c_float, _P]
         L.orc_env_sample_f32.argtypes = [C.c_uint32, C.c_uint32, _P, _P, _P]
+        L.orc_schlick_f32.restype = C.c_float
+        L.orc_schlick_f32.argtypes = [C.c_float, C.c_float]
+        L.orc_reflect_f32.argtypes = [_P, _P, _P]
+        L.orc_refract_f32.argtypes = [_P, _P, C.c_float, _P]
+        L.orc_aabb_hit_f32.restype = C.c_int
+        L.orc_aabb_hit_f32.argtypes = [_P, _P, _P, _P, C.c_float, C.c_float]
+        L.orc_scatter_f32.restype = C.c_int
+        L.orc_scatter_f32.argtypes = [C.POINTER(N.vr_material), _P, _P, _P, _P, C.c_int, _P, C.c_float,
+                                      C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P]
+        L.orc_cast_ray_f32.argtypes = [_P, C.c_float, C.c_float, C.c_uint64, C.c_uint32, _P, _P]
+        L.orc_pixel_ndc_f32.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]
+        L.orc_env_tables_f64.argtypes = [C.c_uint32, C.c_uint32, _P, _P]
+        L.orc_env_pdf_dir_f64.argtypes = [C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P]
+        L.orc_env_sample_dir_f64.argtypes = [C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, _P]
+        L.orc_unit_disc_f32.argtypes = [C.c_uint64, C.c_uint32, _P]
         L.orc_nrc_forward.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P, C.c_float, _P, _P, _P, C.c_int]
         L.orc_nrc_gradients.restype = C.c_double
         L.orc_nrc_gradients.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P, C.c_float, _P, _P, _P, _P,
@@ -144,6 +167,95 @@ def path_words(seed, request, pixel, sample, count):
 def unit_sphere_samples(seed, count):
     out = np.zeros((count, 3), np.float32)
     lib().orc_unit_sphere_f32(seed, count, out.ctypes.data)
+    return out
+
+
+def _f32v(x, n=3):
+    return np.ascontiguousarray(np.asarray(x, np.float32).reshape(n))
+
+
+def schlick(cos_theta, ref_idx):
+    """Dielectric::reflectance (simple.rs:192-197) in f32."""
+    return np.float32(lib().orc_schlick_f32(float(cos_theta), float(ref_idx)))
+
+
+def reflect(v, n):
+    out = np.zeros(3, np.float32)
+    a, b = _f32v(v), _f32v(n)
+    lib().orc_reflect_f32(a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def refract(uv, n, eta):
+    out = np.zeros(3, np.float32)
+    a, b = _f32v(uv), _f32v(n)
+    lib().orc_refract_f32(a.ctypes.data, b.ctypes.data, float(eta), out.ctypes.data)
+    return out
+
+
+def aabb_hit(lo, hi, o, d, t_min=1e-5, t_max=float("inf")):
+    """AABB::hit (aabb.rs:86-148) on one box."""
+    a, b, c, e = _f32v(lo), _f32v(hi), _f32v(o), _f32v(d)
+    return bool(lib().orc_aabb_hit_f32(a.ctypes.data, b.ctypes.data, c.ctypes.data, e.ctypes.data,
+                                       float(t_min), float(t_max)))
+
+
+def scatter(material, ray_o, ray_d, point, normal, front_face=True, uv=(0.0, 0.0), t=1.0, seed=1, pixel=0,
+            sample=0):
+    """Material::scatter of one hit record (simple.rs, traits.rs:19-41) with the
+    random stream (seed, 0, pixel, sample).  Returns (attenuation, ray or None)."""
+    m = N.vr_material(material.kind, N.F3(*material.color), material.param, -1, -1, material.index,
+                      material.roughness, material.metallic, material.emittance, int(material.transparent))
+    ro, rd, pt, nr, uvv = _f32v(ray_o), _f32v(ray_d), _f32v(point), _f32v(normal), _f32v(uv, 2)
+    att = np.zeros(3, np.float32)
+    oo = np.zeros(3, np.float32)
+    od = np.zeros(3, np.float32)
+    has = lib().orc_scatter_f32(C.byref(m), ro.ctypes.data, rd.ctypes.data, pt.ctypes.data, nr.ctypes.data,
+                                int(front_face), uvv.ctypes.data, float(t), seed, pixel, sample,
+                                att.ctypes.data, oo.ctypes.data, od.ctypes.data)
+    return att, ((oo, od) if has else None)
+
+
+def pixel_ndc(width, height, index, flags=0):
+    """Pixel centre -> NDC (iterative.rs:25-33)."""
+    xy = np.zeros(2, np.float32)
+    lib().orc_pixel_ndc_f32(width, height, index, flags, xy.ctypes.data)
+    return xy
+
+
+def env_pdf_cells(image):
+    """FP64 per-cell pdf of the environment importance sampler, (H, W)."""
+    img = np.ascontiguousarray(image, np.float32)
+    h, w = img.shape[:2]
+    out = np.zeros((h, w), np.float64)
+    lib().orc_env_tables_f64(w, h, img.ctypes.data, out.ctypes.data)
+    return out
+
+
+def env_pdf_dirs(image, dirs):
+    """FP64 solid-angle pdf of the environment sampler at the given directions."""
+    img = np.ascontiguousarray(image, np.float32)
+    h, w = img.shape[:2]
+    dv = np.ascontiguousarray(dirs, np.float64)
+    out = np.zeros(len(dv), np.float64)
+    lib().orc_env_pdf_dir_f64(w, h, img.ctypes.data, len(dv), dv.ctypes.data, out.ctypes.data)
+    return out
+
+
+def env_sample_dirs(image, u):
+    """Directions (and their pdf) drawn by the FP64 environment sampler from u (n, 2)."""
+    img = np.ascontiguousarray(image, np.float32)
+    h, w = img.shape[:2]
+    uu = np.ascontiguousarray(u, np.float64)
+    dirs = np.zeros((len(uu), 3), np.float64)
+    pdf = np.zeros(len(uu), np.float64)
+    lib().orc_env_sample_dir_f64(w, h, img.ctypes.data, len(uu), uu.ctypes.data, dirs.ctypes.data, pdf.ctypes.data)
+    return dirs, pdf
+
+
+def unit_disc_samples(seed, count):
+    out = np.zeros((count, 2), np.float32)
+    lib().orc_unit_disc_f32(seed, count, out.ctypes.data)
     return out
 
 
@@ -233,6 +345,13 @@ class OracleScene:
         links = np.zeros((n, 3), np.int64)
         lib().orc_mesh_tree(self.h, mesh, boxes.ctypes.data, links.ctypes.data)
         return boxes, links
+
+    def cast_ray(self, x, y, seed=1, pixel=0):
+        """CameraAcceleration::cast_ray (camera.rs:69-82) at NDC (x, y)."""
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.float32)
+        lib().orc_cast_ray_f32(self.h, float(x), float(y), seed, pixel, o.ctypes.data, d.ctypes.data)
+        return o, d
 
     def render(self, desc, rgba=None, threads=0, row_step=1, prune=False):
         """Returns (rgba, seconds, stats)."""

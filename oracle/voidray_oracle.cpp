@@ -1459,6 +1459,113 @@ void orc_tonemap(const vr_post_desc* p, const float* src, float* dst) {
 
 int orc_num_threads(void) { return hw_threads(0); }
 
+// ---- pieces of the path exported for their closed-form pins (tests only) ----
+
+// Dielectric::reflectance (simple.rs:192-197), reflect / refract (math.rs:12-21).
+float orc_schlick_f32(float c, float idx) { return schlick<float>(c, idx); }
+void orc_reflect_f32(const float* v, const float* n, float* out) {
+    V3<float> r = reflect(load3<float>(v, 0), load3<float>(n, 0));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+void orc_refract_f32(const float* uv, const float* n, float eta, float* out) {
+    V3<float> r = refract(load3<float>(uv, 0), load3<float>(n, 0), eta);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+// AABB::hit (aabb.rs:86-148) on one box.
+int orc_aabb_hit_f32(const float* lo, const float* hi, const float* o, const float* d, float tmin, float tmax) {
+    AABB<float> b{load3<float>(lo, 0), load3<float>(hi, 0)};
+    Ray<float> r{load3<float>(o, 0), load3<float>(d, 0)};
+    return aabb_hit(b, r, tmin, tmax) ? 1 : 0;
+}
+// Material::scatter for one hit record, random stream (seed, 0, pixel,
+// sample) as a render path draws it.  hit: point[3], normal[3] (the record's,
+// already facing the ray), uv[2], t; returns 1 when a scattered ray exists.
+int orc_scatter_f32(const vr_material* mat, const float* ro, const float* rd, const float* hit_point,
+                    const float* hit_normal, int front_face, const float* uv, float t, uint64_t seed, uint32_t pixel,
+                    uint32_t sample, float* att, float* out_o, float* out_d) {
+    Scene<float> sc;
+    Material<float> m{mat->kind, load3<float>(mat->color, 0), mat->param, -1, -1, mat->index, mat->roughness,
+                      mat->metallic, mat->transparent != 0};
+    Ray<float> ray{load3<float>(ro, 0), load3<float>(rd, 0)};
+    Hit<float> h{};
+    h.point = load3<float>(hit_point, 0);
+    h.normal = load3<float>(hit_normal, 0);
+    h.front_face = front_face != 0;
+    h.uv = V2<float>{uv[0], uv[1]};
+    h.t = t;
+    PathRng rng(seed, 0, pixel, sample);
+    RenderCounters rc;
+    Scatter<float> out = scatter(sc, m, ray, h, rng, rc);
+    for (int a = 0; a < 3; ++a) {
+        att[a] = (&out.att.x)[a];
+        out_o[a] = out.has_ray ? (&out.ray.o.x)[a] : 0.f;
+        out_d[a] = out.has_ray ? (&out.ray.d.x)[a] : 0.f;
+    }
+    return out.has_ray ? 1 : 0;
+}
+// CameraAcceleration::cast_ray (camera.rs:69-82) at NDC (x, y); the thin-lens
+// disc sample, if any, is the first draw of stream (seed, 0, pixel, 0).
+void orc_cast_ray_f32(void* sp, float x, float y, uint64_t seed, uint32_t pixel, float* o, float* d) {
+    PathRng rng(seed, 0, pixel, 0);
+    RenderCounters rc;
+    Ray<float> r = cast_ray(*(Scene<float>*)sp, x, y, rng, rc);
+    for (int a = 0; a < 3; ++a) {
+        o[a] = (&r.o.x)[a];
+        d[a] = (&r.d.x)[a];
+    }
+}
+// Pixel -> NDC of iterative.rs:25-33 (the pixel centre, before jitter).
+void orc_pixel_ndc_f32(uint32_t W, uint32_t H, uint32_t index, uint32_t flags, float* xy) {
+    uint32_t x = index % W;
+    uint32_t y = (flags & VR_RENDER_ROWS_BY_HEIGHT) ? index / H : index / W;
+    float dd = (float)std::max(W, H);
+    xy[0] = ((float)(2 * x + 1) - (float)W) / dd;
+    xy[1] = ((float)((uint32_t)(2u * (uint32_t)(H - y) - 1u)) - (float)H) / dd;
+}
+// Environment importance sampling in FP64: per-cell pdf (H*W), the solid-angle
+// pdf of directions, and directions drawn from (u1, u2).
+void orc_env_tables_f64(uint32_t w, uint32_t h, const float* data, double* pdf_cell) {
+    Texture<double> env;
+    vr_texture t{w, h, VR_TEX_RGB32F, VR_SAMPLE_BILINEAR, data};
+    make_texture(env, t);
+    EnvTables<double> T = make_env_tables(env);
+    for (size_t i = 0; i < T.pdf.size(); ++i) pdf_cell[i] = T.pdf[i];
+}
+void orc_env_pdf_dir_f64(uint32_t w, uint32_t h, const float* data, uint32_t n, const double* dirs, double* pdf) {
+    Texture<double> env;
+    vr_texture t{w, h, VR_TEX_RGB32F, VR_SAMPLE_BILINEAR, data};
+    make_texture(env, t);
+    EnvTables<double> T = make_env_tables(env);
+    for (uint32_t i = 0; i < n; ++i) pdf[i] = env_pdf_dir(T, v3<double>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+void orc_env_sample_dir_f64(uint32_t w, uint32_t h, const float* data, uint32_t n, const double* u, double* dirs,
+                            double* pdf) {
+    Texture<double> env;
+    vr_texture t{w, h, VR_TEX_RGB32F, VR_SAMPLE_BILINEAR, data};
+    make_texture(env, t);
+    EnvTables<double> T = make_env_tables(env);
+    for (uint32_t i = 0; i < n; ++i) {
+        V3<double> dv = env_sample_dir(T, u[2 * i], u[2 * i + 1], pdf[i]);
+        dirs[3 * i] = dv.x;
+        dirs[3 * i + 1] = dv.y;
+        dirs[3 * i + 2] = dv.z;
+    }
+}
+// UnitDisc (square rejection), first draw of stream (seed, 0, i, 0).
+void orc_unit_disc_f32(uint64_t seed, uint32_t count, float* out) {
+    uint64_t capped = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        PathRng r(seed, 0, i, 0);
+        V2<float> p = unit_disc<float>(r, capped);
+        out[2 * i] = p.x;
+        out[2 * i + 1] = p.y;
+    }
+}
+
 }  // extern "C"
 
 extern "C" {
