@@ -34,13 +34,47 @@ def _counters_from_profiles():
         return json.load(f)
 
 
-def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
+# Host threads for the CPU baselines: a one-GPU box grants its job 16 host
+# cores (a share of the node's), whatever the node's logical CPU count says, so
+# the CPU legs use at most 16 threads; the node's count and CPU model are
+# recorded beside each baseline.
+CPU_THREADS = 16
+
+
+def host_info():
+    """CPU model and the logical CPUs the OS reports on the host running the bench."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "hardware_concurrency": os.cpu_count(),
+            "threads_used": None, "thread_cap": CPU_THREADS,
+            "thread_cap_reason": "the one-GPU job's share of the host is 16 cores"}
+
+
+def _cpu_threads():
+    import oracle
+    return min(CPU_THREADS, oracle.num_threads())
+
+
+def _host(threads):
+    h = host_info()
+    h["threads_used"] = threads
+    return h
+
+
+def cpu_baseline(scene, o, d, budget_s=12.0):
     """The oracle (reference algorithm restated in C++) on host cores over a
     bounded sample of the same rays: closest-hit rays/s."""
     import numpy as np
 
     import oracle
-    threads = min(max_threads, oracle.num_threads())
+    threads = _cpu_threads()
     t0 = time.time()
     orc = oracle.OracleScene(scene)
     build_s = time.time() - t0
@@ -56,7 +90,8 @@ def cpu_baseline(scene, o, d, budget_s=12.0, max_threads=16):
             chunk *= 2
     return {"value": done / spent / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{done} rays drawn uniformly from the same 4,194,304-ray workload through the same "
-                      f"1,048,576-triangle scene; reference median-split tree built in {build_s:.1f} s (untimed)"}
+                      f"1,048,576-triangle scene; reference median-split tree built in {build_s:.1f} s (untimed)",
+            "host": _host(threads)}
 
 
 _TERRAIN = {}
@@ -72,7 +107,7 @@ def _terrain(ctx):
 
 
 def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=8,
-               warm_size=(240, 135, 2)):
+               warm_size=(240, 135, 2), query_roughness=0.0):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
     + MIS) at width x height, judged against the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
@@ -176,13 +211,15 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
                                        make_render_desc(ww, wh, wspp, wspp, st, sample_offset=0, seed=11 + k, flags=ext,
                                                         rr_depth=4),
                                        train_paths=ww * wh * wspp, train_vertices=4, cache_depth=depth,
-                                       train_steps=train_steps if trained else 0, frame=k, stream=sp)
+                                       train_steps=train_steps if trained else 0, frame=k, stream=sp,
+                                       query_roughness=query_roughness)
                     return 0
                 cache.render_frame(accel, img.data_ptr(), make_render_desc(w, h, spf, frames * spf, st,
                                                                            sample_offset=(k - warm) * spf, seed=7,
                                                                            flags=ext, rr_depth=4),
                                    train_paths=1 << 16, train_vertices=4, cache_depth=depth,
-                                   train_steps=train_steps if trained else 0, frame=k, stream=sp)
+                                   train_steps=train_steps if trained else 0, frame=k, stream=sp,
+                                   query_roughness=query_roughness)
                 return spf
 
             pts = curve(warm + frames, cached_pass, frames * spf)
@@ -211,7 +248,63 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     out["plain_final"] = {"ms": round(plain_pts[-1][0], 2), "relmse": round(plain_pts[-1][2], 5),
                           "mean_over_reference": round(float(plain_img.mean() / ref.mean()), 4),
                           "relmse_8x8": round(relmse(box8(plain_img), ref8), 6)}
+    out["query_roughness"] = query_roughness
+    out["roofline"] = _cache_roofline(ctx, torch, stream, lo, hi)
     return out
+
+
+NRC_FLOP_INFER = 2 * (128 * 64 + 3 * 64 * 64 + 64 * 16)   # 43,008 per query: the MLP forward
+NRC_FLOP_TRAIN = 3 * NRC_FLOP_INFER                          # forward, input gradients, weight gradients
+MFMA_BF16_PEAK_TFLOPS = 2500.0                               # MI355X dense BF16
+
+
+def _cache_roofline(ctx, torch, stream, lo, hi, n_infer=1 << 20, n_train=1 << 16, reps=10):
+    """MFMA roofline of the learned cache's kernels, timed live with HIP events
+    on the stream they run on: inference (one k_nrc_infer launch over n_infer
+    queries) and one training step over n_train samples (k_nrc_train with its
+    grid-gradient atomics, k_nrc_wgrad, the reduction and Adam).  Queries:
+    positions uniform in the scene box, random unit directions and normals,
+    roughness U(0, 1); training targets U(0, 1)."""
+    from voidray_amd.device import RadianceCache
+    sp = stream.cuda_stream
+    g = torch.Generator(device="cuda").manual_seed(5)
+    n = max(n_infer, n_train)
+    lo_t = torch.tensor(lo, dtype=torch.float32, device="cuda")
+    hi_t = torch.tensor(hi, dtype=torch.float32, device="cuda")
+    pos = lo_t + (hi_t - lo_t) * torch.rand((n, 3), device="cuda", generator=g)
+    dirs = torch.nn.functional.normalize(torch.randn((n, 3), device="cuda", generator=g), dim=1)
+    nrm = torch.nn.functional.normalize(torch.randn((n, 3), device="cuda", generator=g), dim=1)
+    rough = torch.rand(n, device="cuda", generator=g)
+    target = torch.rand((n, 3), device="cuda", generator=g)
+    out = torch.empty((n, 3), device="cuda")
+    torch.cuda.synchronize()
+    cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=2048)
+    args = (pos.data_ptr(), dirs.data_ptr(), nrm.data_ptr(), rough.data_ptr())
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_inf = timed(lambda: cache.infer(n_infer, *args, out.data_ptr(), stream=sp))
+    ms_tr = timed(lambda: cache.train(n_train, *args, target.data_ptr(), stream=sp))
+    cache.close()
+    tf_inf = NRC_FLOP_INFER * n_infer / (ms_inf * 1e-3) / 1e12
+    tf_tr = NRC_FLOP_TRAIN * n_train / (ms_tr * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": "k_nrc_infer", "achieved": round(tf_inf, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tf_inf / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None,
+            "infer_ms": round(ms_inf, 4), "queries": n_infer, "flop_per_query": NRC_FLOP_INFER,
+            "grid_gather_bytes_per_query": 512,
+            "train_step": {"ms": round(ms_tr, 4), "samples": n_train, "flop_per_sample": NRC_FLOP_TRAIN,
+                           "achieved_tflops": round(tf_tr, 2), "frac": round(tf_tr / MFMA_BF16_PEAK_TFLOPS, 5),
+                           "kernels": "k_nrc_train + k_nrc_wgrad + k_nrc_reduce + k_adam + k_adam_grid4"}}
 
 
 def _render_roofline(st, env_cdf=False):
@@ -270,11 +363,12 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
            "exact_engine_rays": st["fallback_rays"], "roofline": _render_roofline(st)}
     if cpu:
         import oracle
-        threads = min(16, oracle.num_threads())
+        threads = _cpu_threads()
         ref, sec, _ = oracle.OracleScene(c1).render(d1, threads=threads)
         n = 512 * 512 * 16
         leg["cpu_baseline"] = {"samples_per_s": round(n / sec, 1), "cores": threads, "kind": "port",
-                               "sample": "the full frame (512x512, 16 spp), reference estimator and traversal"}
+                               "sample": "the full frame (512x512, 16 spp), reference estimator and traversal",
+                               "host": _host(threads)}
         leg["gpu_over_cpu"] = round(leg["samples_per_s"] / (n / sec), 1)
         leg["precision_vs_f32_restatement"] = _precision(img, ref)
     out["cornell_512_16spp"] = leg
@@ -293,14 +387,15 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
            "roofline": _render_roofline(st, env_cdf=True)}
     if cpu:
         import oracle
-        threads = min(16, oracle.num_threads())
+        threads = _cpu_threads()
         orc = oracle.OracleScene(c3)
         # bounded sample: 1 spp on every 3rd row (360 rows x 1920 pixels), same estimator, reference traversal
         d1 = make_render_desc(1920, 1080, 1, c3_spp, s3, seed=5, flags=ext, rr_depth=4)
         ref, sec, ost = orc.render(d1, threads=threads, row_step=3)
         n = 1920 * 360
         leg["cpu_baseline"] = {"samples_per_s": round(n / sec, 1), "cores": threads, "kind": "port",
-                               "sample": f"{n} samples (1 spp, every 3rd row), reference median-split traversal"}
+                               "sample": f"{n} samples (1 spp, every 3rd row), reference median-split traversal",
+                               "host": _host(threads)}
         leg["gpu_over_cpu"] = round(leg["samples_per_s"] / (n / sec), 1)
         # precision: the same 1-spp frame on the GPU against the f32 restatement on
         # those rows, and against the FP64 oracle on every 30th row
@@ -329,7 +424,32 @@ def _pct(x, q):
     return round(float(np.percentile(np.asarray(x), q)), 3)
 
 
-def _serving_leg(ctx, torch, stream):
+def _serving_cpu_baseline(pool, reqs, flags, row_step=64):
+    """The 64 requests rendered one at a time by the oracle on the host, each
+    on every `row_step`-th row at its full spp (the same estimator, the
+    reference median-split traversal): samples/s over the rendered samples.
+    Oracle scene builds are not timed."""
+    import oracle
+    from voidray_amd.device import make_render_desc
+    from voidray_amd.scene import RenderSettings
+    threads = _cpu_threads()
+    t0 = time.time()
+    orcs = [oracle.OracleScene(s) for s in pool]
+    build_s = time.time() - t0
+    spent, n = 0.0, 0
+    for i, (si, w, h, spp, seed) in enumerate(reqs):
+        st = RenderSettings(total_samples=spp, max_bounces=16, firefly_clamp=float("inf"))
+        _, sec, _ = orcs[si].render(make_render_desc(w, h, spp, spp, st, seed=seed, request=i, flags=flags,
+                                                     rr_depth=4), threads=threads, row_step=row_step)
+        spent += sec
+        n += w * len(range(0, h, row_step)) * spp
+    return {"samples_per_s": round(n / spent, 1), "cores": threads, "kind": "port", "seconds": round(spent, 2),
+            "sample": f"all 64 requests one at a time, every {row_step}th row at full spp ({n} samples); "
+                      f"oracle scene builds ({build_s:.1f} s) not timed",
+            "host": _host(threads)}
+
+
+def _serving_leg(ctx, torch, stream, cpu=True):
     """64 render requests (resolutions 256^2..1920x1080, 1-32 spp, own seeds)
     over eight scenes of 10k-2M triangles; depth 16, RR from 4, env IS + MIS.
     Batched: one captured hipGraph launch with the autotuned chunk/block size
@@ -411,6 +531,8 @@ def _serving_leg(ctx, torch, stream):
     svc_s = [evs[k].elapsed_time(evs[k + 1]) for k in range(len(reqs))]
     seq_ms = lat_s[-1]
     sums_s = torch.stack([t.double().sum() for t in targets]).cpu()
+    roof = _render_roofline(dict(bst, kernel_ms=batch_ms), env_cdf=True)
+    cpu_leg = _serving_cpu_baseline(pool, reqs, flags & ~N.VR_RENDER_OVERWRITE) if cpu else None
     return {
         "requests": len(reqs), "samples": samples, "scenes_triangles": [s.num_triangles for s in pool],
         "scene_build_s": round(build_s, 2), "settings": "depth 16, RR from 4, env IS + MIS",
@@ -425,6 +547,9 @@ def _serving_leg(ctx, torch, stream):
                        "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
                        "service_p50_ms": _pct(svc_s, 50), "service_p99_ms": _pct(svc_s, 99)},
         "batched_over_sequential": round(seq_ms / batch_ms, 3),
+        "roofline": roof,
+        "cpu_baseline": cpu_leg,
+        "gpu_over_cpu": (round(samples / (batch_ms * 1e-3) / cpu_leg["samples_per_s"], 1) if cpu_leg else None),
         "images_equal_checksums": bool(torch.equal(sums_b, sums_s)),
         "nonfinite_values": nonfinite,
     }
@@ -443,6 +568,9 @@ def main():
     ap.add_argument("--only-serving", action="store_true", help="run the batched-serving leg alone (prints it)")
     ap.add_argument("--no-cache", action="store_true", help="skip the learned-radiance-cache leg")
     ap.add_argument("--only-cache", action="store_true", help="run the learned-radiance-cache leg alone (prints it)")
+    ap.add_argument("--cache-query-roughness", type=float, default=0.0,
+                    help="cached paths end at the first vertex from the query depth on whose roughness input is "
+                         "at least this (0: at the query depth whatever the material)")
     ap.add_argument("--cache-train-steps", type=int, default=8,
                     help="Adam steps per cached frame (disjoint slices of the frame's 2^16 samples)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -465,13 +593,14 @@ def main():
     if args.only_cache:
         st = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(st)
-        print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st, train_steps=args.cache_train_steps)}),
+        print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st, train_steps=args.cache_train_steps,
+                                                       query_roughness=args.cache_query_roughness)}),
               flush=True)
         return
     if args.only_serving:
         st = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(st)
-        print(json.dumps({"serving": _serving_leg(ctx, torch, st)}), flush=True)
+        print(json.dumps({"serving": _serving_leg(ctx, torch, st, cpu=not args.no_cpu)}), flush=True)
         return
     scene = W.random_triangle_scene(args.tris)
     o, d = W.microbench_rays(args.rays)
@@ -546,8 +675,9 @@ def main():
     pmc = _counters_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
-    serving = None if args.no_serving else _serving_leg(ctx, torch, stream)
-    cache = None if args.no_cache else _cache_leg(ctx, torch, stream, train_steps=args.cache_train_steps)
+    serving = None if args.no_serving else _serving_leg(ctx, torch, stream, cpu=not args.no_cpu)
+    cache = None if args.no_cache else _cache_leg(ctx, torch, stream, train_steps=args.cache_train_steps,
+                                                  query_roughness=args.cache_query_roughness)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),

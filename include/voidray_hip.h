@@ -23,7 +23,6 @@ extern "C" {
 #define VR_ERR_HIP (-2)         /* HIP runtime failure                     */
 #define VR_ERR_OOM (-3)         /* device allocation failed                */
 #define VR_ERR_UNSUPPORTED (-4) /* feature not available in this build     */
-#define VR_ERR_CANCELLED (-5)   /* host cancel flag observed between passes */
 
 typedef struct vr_ctx vr_ctx;
 typedef struct vr_scene vr_scene;
@@ -208,6 +207,13 @@ int vr_scene_get_stats(const vr_scene* scene, vr_scene_stats* out);
 /* traversal engines */
 #define VR_ENGINE_WIDE 0      /* quantised BVH4 + robust-winner check + exact fallback */
 #define VR_ENGINE_REFERENCE 1 /* flattened reference trees, both children, no pruning  */
+/* The wide BVH with the watertight ray/triangle test (Woop, Benthin & Wald,
+ * JCGT 2013) in place of Moller-Trumbore: no ray slips between the triangles
+ * of a closed mesh.  No counterpart in the reference (Triangle::hit,
+ * mesh.rs:144-188, is Moller-Trumbore): hits are not the reference's, so
+ * renders accept it only with the extended estimator (VR_RENDER_ENV_IS or
+ * VR_RENDER_RUSSIAN_ROULETTE). */
+#define VR_ENGINE_WATERTIGHT 2
 
 typedef struct vr_trace_desc {
     uint32_t num_rays;
@@ -383,10 +389,11 @@ typedef struct vr_cache_info {
 typedef struct vr_cache_frame {
     uint32_t train_paths;     /* expected training paths per frame, drawn at random among its paths */
     uint32_t train_vertices;  /* vertices per training path recorded as samples (<= 8) */
-    uint32_t cache_depth;     /* other paths query the cache at this vertex (2 = after bounce 2) */
-    uint32_t train_steps;     /* Adam steps per frame, each on one slice of the frame's samples */
-    uint32_t frame;           /* frame index: varies the choice of training paths */
-    uint32_t pad;
+    uint32_t cache_depth;     /* other paths query the cache at this vertex (2 = after bounce 2) ... */
+    uint32_t train_steps;     /* Adam steps per frame (<= 32), each on one random mini-batch of the frame's samples */
+    uint32_t frame;           /* frame index: varies the choice of training paths and mini-batches */
+    float query_roughness;    /* ... or at the first vertex from cache_depth on whose roughness input (1 diffuse,
+                                 the fuzz of Metal, 0 Dielectric) is at least this; 0: at cache_depth always */
 } vr_cache_frame;
 
 typedef struct vr_cache_stats {

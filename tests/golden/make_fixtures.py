@@ -69,8 +69,79 @@ def c2_full():
         f.write("\n")
 
 
+def scene_digest(scene):
+    """SHA-256 of everything a Scene hands to the library: meshes (positions,
+    normals, uvs, indices), surfaces, objects, materials, analytic surfaces,
+    textures, the environment and the camera, in a fixed order."""
+    import hashlib
+    h = hashlib.sha256()
+
+    def arr(a, dt):
+        if a is None:
+            h.update(b"none")
+        else:
+            a = np.ascontiguousarray(a, dt)
+            h.update(str(a.shape).encode())
+            h.update(a.tobytes())
+
+    for m in scene.meshes:
+        arr(m.positions, "<f4")
+        arr(m.normals, "<f4")
+        arr(m.uvs, "<f4")
+        arr(m.indices, "<u4")
+    arr(np.asarray(scene.surfaces, np.uint64), "<u8")
+    arr(np.asarray(scene.objects, np.int64).reshape(-1, 2), "<i8")
+    for mt in scene.materials:
+        h.update(repr((mt.kind, tuple(np.float32(mt.color).tolist()), float(np.float32(mt.param)), mt.albedo_texture,
+                       mt.normal_texture, float(np.float32(mt.index)), float(np.float32(mt.roughness)),
+                       float(np.float32(mt.metallic)), float(np.float32(mt.emittance)), bool(mt.transparent))).encode())
+    for a in scene.analytic:
+        h.update(repr((a.kind, tuple(np.float32(a.center).tolist()), float(np.float32(a.radius)),
+                       float(np.float32(a.height)))).encode())
+    for t in scene.textures:
+        arr(t.data, t.data.dtype.newbyteorder("<"))
+        h.update(repr((t.format, t.sample_type)).encode())
+    e = scene.environment
+    if e is None:
+        h.update(b"no environment")
+    else:
+        h.update(repr((e.kind, tuple(np.float32(e.color).tolist()))).encode())
+        arr(e.image, "<f4")
+    c = scene.camera
+    arr(np.concatenate([c.eye, c.direction, c.up, [c.fov]]).astype(np.float32), "<f4")
+    h.update(repr(c.dof).encode())
+    return h.hexdigest()
+
+
+def generator_cases():
+    """The procedural workloads: configs 1-4 at full size and the scenes the tests use."""
+    return {
+        "cornell_512": lambda: W.cornell_room(),
+        "terrain_1080p": lambda: W.terrain_spheres(),
+        "material_spheres": lambda: W.material_spheres(),
+        "analytic_spheres": lambda: W.analytic_spheres(),
+        "analytic_and_meshes": lambda: W.analytic_and_meshes(),
+        "microfacet_spheres": lambda: W.microfacet_spheres(),
+        "cornell_demo": lambda: W.cornell_demo(),
+        "material_demo": lambda: W.material_demo(),
+        "cornell_zero_normal_quads": lambda: W.cornell_zero_normal_quads(),
+    }
+
+
+def generator_digests():
+    """SHA-256 of every procedural generator's output (scenes, the serving
+    pool and request list); the microbench inputs are in c2_full_digests.json."""
+    import json
+    doc = {name: scene_digest(make()) for name, make in generator_cases().items()}
+    doc["serving_pool"] = [scene_digest(s) for s in W.serving_scene_pool()]
+    doc["serving_requests"] = [list(r) for r in W.serving_requests()]
+    with open(os.path.join(HERE, "generator_digests.json"), "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["c2_small", "c2_full"]
+    which = sys.argv[1:] or ["c2_small", "c2_full", "generator_digests"]
     for name in which:
         globals()[name]()
     print("ok")

@@ -23,3 +23,18 @@ def test_full_fixture_inputs_and_sample_blocks():
         sl = slice(blk * B, (blk + 1) * B)
         r = orc.trace(o[sl], d[sl])
         assert c2_full_digests(r["prim"], r["t"], r["u"], r["v"], B) == [fx["blocks"][blk]], blk
+
+
+def test_procedural_generators_reproduce_their_digests():
+    """Every procedural workload (configs 1-4 at full size: the cornell room,
+    the 1080p terrain with 200 spheres, textures and HDR sky, the 8-scene
+    serving pool and its 64 requests; and the scenes the tests render) gives
+    the committed SHA-256 of everything it hands to the library."""
+    from tests.golden.make_fixtures import generator_cases, scene_digest
+    from voidray_amd import workloads as W
+    with open("tests/golden/generator_digests.json") as f:
+        fx = json.load(f)
+    for name, make in generator_cases().items():
+        assert scene_digest(make()) == fx[name], name
+    assert [scene_digest(s) for s in W.serving_scene_pool()] == fx["serving_pool"]
+    assert [list(r) for r in W.serving_requests()] == fx["serving_requests"]

@@ -150,3 +150,48 @@ def test_leaf_reference_layout(gpu_ctx, oracle_lib):
     _compare(ref, {k: v[sel] for k, v in wide.items()}, "refs/sample-vs-oracle")
     anyh = acc.hit(o, d, any_hit=True)["hit"]
     assert np.array_equal(anyh.astype(bool), wide["prim"] != 0xFFFFFFFF)
+
+
+def test_watertight_engine(gpu_ctx):
+    """VR_ENGINE_WATERTIGHT on the GPU gives the host run of the same
+    traversal code bit for bit (closest and any hit), loses no ray through a
+    closed sphere, and renders only under the extended estimator."""
+    from tests.test_watertight import _scene, _sphere_targets
+    from voidray_amd.device import make_render_desc, trace_host
+    from voidray_amd.scene import RenderSettings
+    mesh = W.uv_sphere((0.1, -0.2, 0.3), 1.0, 48, 24)
+    sphere = _scene(mesh)
+    c = np.array([0.1, -0.2, 0.3])
+    tgt = _sphere_targets(mesh)
+    src = c + np.array([0.13, -0.07, 0.05])
+    o = np.tile(src, (len(tgt), 1)).astype(np.float32)
+    d = W._normalize_rows((tgt - src).astype(np.float32))
+    cases = [("sphere", sphere, o, d), ("soup", W.random_triangle_scene(20000, seed=5)) + W.microbench_rays(8192, res=64)]
+    for name, scene, oo, dd in cases:
+        announce(f"watertight {name} n={len(oo)}")
+        acc = scene.build_acceleration(gpu_ctx)
+        got = acc.hit(oo, dd, engine=N.VR_ENGINE_WATERTIGHT)
+        ref = trace_host(scene, oo, dd, engine=N.VR_ENGINE_WATERTIGHT)
+        _compare(ref, got, f"watertight/{name}")
+        anyh = acc.hit(oo, dd, any_hit=True, engine=N.VR_ENGINE_WATERTIGHT)["hit"]
+        assert np.array_equal(anyh.astype(bool), got["prim"] != 0xFFFFFFFF)
+        if name == "sphere":
+            assert np.all(got["prim"] != 0xFFFFFFFF)
+    # renders: extended estimator only
+    scene = W.terrain_spheres(grid=24, n_spheres=6, sphere_res=(16, 8), tex_size=64, env_size=(64, 32), sun_peak=200.0)
+    acc = scene.build_acceleration(gpu_ctx)
+    st = RenderSettings(total_samples=16, max_bounces=8, firefly_clamp=float("inf"))
+    ext = N.VR_RENDER_ENV_IS | N.VR_RENDER_RUSSIAN_ROULETTE | N.VR_RENDER_ALPHA_PER_CALL
+    imgs = {}
+    for eng in (N.VR_ENGINE_WIDE, N.VR_ENGINE_WATERTIGHT):
+        img = np.zeros((24, 32, 4), np.float32)
+        acc.render(img, make_render_desc(32, 24, 16, 16, st, seed=3, flags=ext, rr_depth=4, engine=eng))
+        imgs[eng] = img
+    x, r = imgs[N.VR_ENGINE_WATERTIGHT][..., :3], imgs[N.VR_ENGINE_WIDE][..., :3]
+    assert np.isfinite(x).all()
+    relmse = float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+    print(f"[watertight] render vs Moller-Trumbore, same streams: relMSE {relmse:.3e}", flush=True)
+    assert relmse < 1e-3
+    with pytest.raises(N.VoidrayError, match="watertight"):
+        acc.render(np.zeros((4, 4, 4), np.float32),
+                   make_render_desc(4, 4, 1, 1, RenderSettings(total_samples=1), engine=N.VR_ENGINE_WATERTIGHT))

@@ -19,7 +19,7 @@ VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF, VR_BUILD_LEAF_RECORDS, VR_BUILD_LEAF_REFS
 VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
 VR_SURFACE_ANALYTIC = 0x80000000
 VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
-VR_ENGINE_WIDE, VR_ENGINE_REFERENCE = 0, 1
+VR_ENGINE_WIDE, VR_ENGINE_REFERENCE, VR_ENGINE_WATERTIGHT = 0, 1, 2
 VR_MODE_FULL, VR_MODE_NORMAL = 0, 1
 VR_RENDER_ROWS_BY_HEIGHT = 1
 VR_RENDER_ENV_IS = 2
@@ -143,7 +143,7 @@ class vr_cache_info(C.Structure):
 
 class vr_cache_frame(C.Structure):
     _fields_ = [("train_paths", C.c_uint32), ("train_vertices", C.c_uint32), ("cache_depth", C.c_uint32),
-                ("train_steps", C.c_uint32), ("frame", C.c_uint32), ("pad", C.c_uint32)]
+                ("train_steps", C.c_uint32), ("frame", C.c_uint32), ("query_roughness", C.c_float)]
 
 
 class vr_cache_stats(C.Structure):

@@ -164,7 +164,8 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
     return guarded([&] {
         if (!sd || !desc) return fail(VR_ERR_INVALID, "null argument");
         if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
-        if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE)
+        if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE &&
+            desc->engine != VR_ENGINE_WATERTIGHT)
             return fail(VR_ERR_INVALID, "bad engine");
         const uint32_t n = desc->num_rays;
         if (n && (!o || !d)) return fail(VR_ERR_INVALID, "null ray buffers");
@@ -184,16 +185,23 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
         for (uint32_t i = 0; i < n; ++i) {
             const vf3 oi = mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), di = mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
             vr::HitOut h;
+            const bool wt = desc->engine == VR_ENGINE_WATERTIGHT;
             bool exact = desc->engine == VR_ENGINE_REFERENCE;
             if (!exact) {
-                int r = any ? vr::trace_wide<kStack, true>(sv, oi, di, desc->t_min, kInf, ws, h)
+                int r;
+                if (wt)
+                    r = any ? vr::trace_wide<kStack, true, true>(sv, oi, di, desc->t_min, kInf, ws, h)
+                            : vr::trace_wide<kStack, false, true>(sv, oi, di, desc->t_min, kInf, ws, h);
+                else
+                    r = any ? vr::trace_wide<kStack, true>(sv, oi, di, desc->t_min, kInf, ws, h)
                             : vr::trace_wide<kStack, false>(sv, oi, di, desc->t_min, kInf, ws, h);
                 if (r == vr::kStepFallback) {
                     exact = true;
                     ++fb;
                 }
             }
-            if (exact) vr::trace_exact(sv, oi, di, desc->t_min, h);  // any hit: t_max = inf, so any hit counts
+            // any hit: t_max = inf, so any hit counts
+            if (exact) wt ? vr::trace_exact<true>(sv, oi, di, desc->t_min, h) : vr::trace_exact(sv, oi, di, desc->t_min, h);
             if (prim) prim[i] = h.prim;
             if (t) t[i] = h.t;
             if (u) u[i] = h.u;

@@ -170,7 +170,7 @@ struct vr_scene {
     uint32_t fallback_cap = 0;
     u32x2* spill = nullptr;             // per-lane spill stacks of the persistent traversal
     uint32_t spill_lanes = 0;
-    uint32_t persist_blocks[2] = {0, 0};  // resident blocks of the persistent kernels
+    uint32_t persist_blocks[4] = {0, 0, 0, 0};  // resident blocks of the persistent kernels [any + 2 * watertight]
     hipEvent_t trace_ev[2] = {nullptr, nullptr};  // around the wide kernel of the last vr_trace*
     bool trace_timed = false;
     uint32_t last_fallbacks = 0;

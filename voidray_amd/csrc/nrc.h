@@ -65,6 +65,13 @@ struct vr_cache {
     uint32_t* counts = nullptr;
     // frame buffers: training samples and cache queries
     vr::NrcSamples tr{}, qr{};
+    // the frame's training samples reordered for its Adam steps: each sample
+    // draws a mini-batch at random, and inside a mini-batch samples follow the
+    // Morton order of their positions (pos, dir, nrm, rough, target only)
+    vr::NrcSamples trs{};
+    uint32_t *sort_keys = nullptr, *sort_vals = nullptr;  // [2][cap] each: input and output of the sort
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
     std::vector<void*> frame_allocs;
     // pinned read-back: [0] train count, [1] query count, [2] mean loss (f32 bits)
     uint32_t* host_rb = nullptr;
@@ -84,5 +91,11 @@ int nrc_train_step(vr_cache* c, uint32_t n, const uint32_t* d_n, const float* po
                    const float* rough, const float* target, bool update, hipStream_t st, uint32_t part = 0,
                    uint32_t parts = 1);
 int nrc_ensure_frame_buffers(vr_cache* c, uint32_t train_cap, uint32_t query_cap);
+// Fills c->trs from c->tr: sample i goes to mini-batch hash(i, frame) % parts,
+// mini-batches in order, Morton order of the position inside each.  Equal
+// slices of the result are then independent random mini-batches whose
+// neighbouring samples share hash-grid cells.  parts <= kNrcMaxParts.
+constexpr uint32_t kNrcMaxParts = 32;
+int nrc_order_samples(vr_cache* c, uint32_t parts, uint32_t frame, hipStream_t st);
 
 }  // namespace vr

@@ -29,6 +29,8 @@
 #include <cstring>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "nrc.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -337,23 +339,45 @@ __device__ __forceinline__ void nrc_mlp_body(const MlpArgs& a, __bf16* W) {
         store_rows(a.dT, cs, s, 64, d1, q);
         store_rows(a.dT, cs, s, 128, d2, q);
         store_rows(a.dT, cs, s, 192, d3, q);
-        // d(grid features) = W0^T delta_0, input units 0..31
+        // d(grid features) = W0^T delta_0, input units 0..31.  The 16 lanes
+        // of one q hold the same level for 16 samples; when all 16 touch the
+        // same grid entry (neighbouring samples in a coarse cell: the
+        // training samples come in Morton order), their contributions are
+        // summed across the lanes and one lane adds the sum, instead of 16
+        // same-address atomics.
         f32x4 dx[2];
         bwd_hidden(W + kL0, kS0, 2, d0, dx, col, q);
-        if (valid) {
-            for (int t = 0; t < 2; ++t)
-                for (int lv = 0; lv < 2; ++lv) {
-                    const int l = 8 * t + 2 * q + lv;
-                    const float g0 = dx[t][2 * lv], g1 = dx[t][2 * lv + 1];
-                    uint32_t idx[8];
-                    float w[8];
-                    level_corners(a.v, l, e.p, idx, w);
-                    for (int k = 0; k < 8; ++k) {
-                        unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k], w[k] * g0);
-                        unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k] + 1, w[k] * g1);
+        const int lead = lane & ~15;
+        for (int t = 0; t < 2; ++t)
+            for (int lv = 0; lv < 2; ++lv) {
+                const int l = 8 * t + 2 * q + lv;
+                const float g0 = valid ? dx[t][2 * lv] : 0.f, g1 = valid ? dx[t][2 * lv + 1] : 0.f;
+                uint32_t idx[8];
+                float w[8];
+                level_corners(a.v, l, e.p, idx, w);
+                for (int k = 0; k < 8; ++k) {
+                    float c0 = w[k] * g0, c1 = w[k] * g1;
+                    const uint32_t li = (uint32_t)__shfl((int)idx[k], lead);
+                    // an invalid lane contributes zero wherever it is counted
+                    const uint64_t same = __ballot(!valid || idx[k] == li);
+                    const bool merged = ((same >> lead) & 0xFFFFull) == 0xFFFFull;
+                    if (__ballot(merged)) {  // some group of the wave merges: every lane takes part in the sums
+                        for (int off = 8; off >= 1; off >>= 1) {
+                            c0 += __shfl_xor(c0, off);
+                            c1 += __shfl_xor(c1, off);
+                        }
+                    }
+                    if (merged) {
+                        if (lane == lead && valid) {  // an invalid leader: the whole tile is past the count
+                            unsafeAtomicAdd(a.grid_grad + 2 * (size_t)li, c0);
+                            unsafeAtomicAdd(a.grid_grad + 2 * (size_t)li + 1, c1);
+                        }
+                    } else if (valid) {
+                        unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k], c0);
+                        unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k] + 1, c1);
                     }
                 }
-        }
+            }
     }
 }
 
@@ -506,6 +530,58 @@ __global__ void k_nrc_f32(MlpArgs a, int train, float* w_grad) {
             unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k], w[k] * g0);
             unsafeAtomicAdd(a.grid_grad + 2 * (size_t)idx[k] + 1, w[k] * g1);
         }
+    }
+}
+
+// Sort keys of a frame's training samples: bits 27..31 the sample's random
+// mini-batch (hash of its index and the frame, uniform over `parts`), bits
+// 0..26 the Morton code of its position on a 512^3 grid over the cache box;
+// samples past the count sort last.
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {  // 9 bits -> every third bit
+    x &= 0x1FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__global__ void k_nrc_sort_keys(NrcView v, const float* pos, const uint32_t* count, uint32_t cap, uint32_t parts,
+                                uint32_t frame, uint32_t* keys, uint32_t* vals) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (i < n) {
+            uint32_t h = i * 0x9E3779B1u ^ (frame + 0x7F4A7C15u) * 0x85EBCA77u;
+            h ^= h >> 15;
+            h *= 0x2C1B3C6Du;
+            h ^= h >> 12;
+            h *= 0x297A2D39u;
+            h ^= h >> 15;
+            const uint32_t mb = (uint32_t)(((uint64_t)h * parts) >> 32);
+            uint32_t m = 0;
+            for (int a = 0; a < 3; ++a) {
+                const float p = clamp01((pos[3 * (size_t)i + a] - v.origin[a]) * v.inv_extent);
+                m |= spread3(min((uint32_t)(p * 512.f), 511u)) << a;
+            }
+            key = (mb << 27) | m;
+        }
+        keys[i] = key;
+        vals[i] = i;
+    }
+}
+
+// dst[k] = src[order[k]] for the first count samples (the fields training reads).
+__global__ void k_nrc_gather(NrcSamples src, NrcSamples dst, const uint32_t* order, uint32_t cap) {
+    const uint32_t n = min(*src.count, cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const size_t i = order[k];
+        for (int a = 0; a < 3; ++a) {
+            dst.pos[3 * (size_t)k + a] = src.pos[3 * i + a];
+            dst.dir[3 * (size_t)k + a] = src.dir[3 * i + a];
+            dst.nrm[3 * (size_t)k + a] = src.nrm[3 * i + a];
+            dst.target[3 * (size_t)k + a] = src.target[3 * i + a];
+        }
+        dst.rough[k] = src.rough[i];
     }
 }
 
@@ -716,11 +792,57 @@ int nrc_ensure_frame_buffers(vr_cache* c, uint32_t train_cap, uint32_t query_cap
         S = T;  // the old buffers stay in frame_allocs until the cache is destroyed
         return 0;
     };
+    const uint32_t old_cap = c->tr.cap;
     int rc = grow(c->tr, train_cap);
     if (rc) return rc;
+    if (c->tr.cap != old_cap) {
+        // sorted copies of the training samples, the sort's keys and values and its scratch
+        const size_t n = std::max(c->tr.cap, 1u);
+        NrcSamples T{};
+        void* p[7] = {};
+        const size_t bytes[7] = {12 * n, 12 * n, 12 * n, 4 * n, 12 * n, 8 * n, 8 * n};
+        for (int k = 0; k < 7; ++k) {
+            if (hipMalloc(&p[k], bytes[k]) != hipSuccess) return fail(VR_ERR_OOM, "radiance cache sort buffers");
+            c->frame_allocs.push_back(p[k]);
+        }
+        T.pos = (float*)p[0];
+        T.dir = (float*)p[1];
+        T.nrm = (float*)p[2];
+        T.rough = (float*)p[3];
+        T.target = (float*)p[4];
+        T.count = c->tr.count;
+        T.cap = c->tr.cap;
+        c->trs = T;
+        c->sort_keys = (uint32_t*)p[5];
+        c->sort_vals = (uint32_t*)p[6];
+        size_t temp = 0;
+        VR_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, temp, c->sort_keys, c->sort_keys + n, c->sort_vals,
+                                               c->sort_vals + n, c->tr.cap, 0, 32, (hipStream_t)0));
+        if (hipMalloc(&c->sort_temp, std::max<size_t>(temp, 256)) != hipSuccess)
+            return fail(VR_ERR_OOM, "radiance cache sort scratch");
+        c->frame_allocs.push_back(c->sort_temp);
+        c->sort_temp_bytes = std::max<size_t>(temp, 256);
+    }
     rc = ensure_train_scratch(c, train_cap);
     if (rc) return rc;
     return grow(c->qr, query_cap);
+}
+
+int nrc_order_samples(vr_cache* c, uint32_t parts, uint32_t frame, hipStream_t st) {
+    if (parts == 0 || parts > kNrcMaxParts) return fail(VR_ERR_INVALID, "train_steps must be in [1, 32]");
+    const uint32_t cap = c->tr.cap;
+    if (cap == 0 || !c->sort_keys) return fail(VR_ERR_INVALID, "training sample buffers not allocated");
+    const size_t n = cap;
+    hipLaunchKernelGGL(k_nrc_sort_keys, dim3(grid_launch(cap)), dim3(256), 0, st, c->view, (const float*)c->tr.pos,
+                       (const uint32_t*)c->tr.count, cap, parts, frame, c->sort_keys, c->sort_vals);
+    VR_HIP_CHECK(hipGetLastError());
+    size_t temp = c->sort_temp_bytes;
+    VR_HIP_CHECK(rocprim::radix_sort_pairs(c->sort_temp, temp, c->sort_keys, c->sort_keys + n, c->sort_vals,
+                                           c->sort_vals + n, cap, 0, 32, st));
+    hipLaunchKernelGGL(k_nrc_gather, dim3(grid_launch(cap)), dim3(256), 0, st, c->tr, c->trs,
+                       (const uint32_t*)(c->sort_vals + n), cap);
+    VR_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

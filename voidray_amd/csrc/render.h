@@ -11,6 +11,7 @@ struct CacheReq {
     uint32_t train_vertices;   // recorded vertices per training path
     uint32_t train_threshold;  // fraction of training paths * 2^32
     uint32_t frame;
+    float query_roughness;     // query at the first vertex from cache_depth on with roughness input >= this
 };
 // Plans, uploads and enqueues a batch (optionally captured into a graph and launched).
 int render_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr_tuning* tuning, uint32_t flags,

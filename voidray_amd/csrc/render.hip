@@ -93,6 +93,7 @@ struct ReqDev {
     uint32_t train_vertices;  // recorded vertices per training path
     uint32_t train_threshold; // a path trains iff train_hash < threshold (threshold / 2^32 = fraction)
     uint32_t frame;
+    float query_roughness;    // query at the first vertex from cache_depth on with cache_roughness >= this
 };
 
 struct SegDev {
@@ -372,7 +373,8 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                     const bool train_path =
                         r.cmode == kCacheOn &&
                         train_hash(a.path_pix[p], a.path_smp[p], r.request, r.frame) < r.train_threshold;
-                    const bool query = r.cmode == kCacheOn && !train_path && depth == r.cache_depth;
+                    const bool query = r.cmode == kCacheOn && !train_path && depth >= r.cache_depth &&
+                                       cache_roughness(m) >= r.query_roughness;
                     if (query || (train_path && depth < r.train_vertices)) {
                         // the vertex, its outgoing direction and the path state before its contributions
                         vf3 ns = m.normal_tex >= 0 ? tex_sample(sv, m.normal_tex, hit.uv.x, hit.uv.y) : hit.normal;
@@ -615,8 +617,12 @@ int validate(const vr_scene* s, const vr_render_desc& d, uint32_t chunk_paths) {
     if (d.total_samples == 0) return fail(VR_ERR_INVALID, "total_samples must be > 0");
     if (d.max_bounces > kMaxBouncesLimit) return fail(VR_ERR_INVALID, "max_bounces above 64");
     if ((uint64_t)d.width * d.height >= (1ull << 32)) return fail(VR_ERR_INVALID, "image too large");
-    if (d.engine != VR_ENGINE_WIDE && d.engine != VR_ENGINE_REFERENCE) return fail(VR_ERR_INVALID, "bad engine");
+    if (d.engine != VR_ENGINE_WIDE && d.engine != VR_ENGINE_REFERENCE && d.engine != VR_ENGINE_WATERTIGHT)
+        return fail(VR_ERR_INVALID, "bad engine");
     bool ext = is_ext(d);
+    if (d.engine == VR_ENGINE_WATERTIGHT && !ext)
+        return fail(VR_ERR_INVALID, "the watertight engine renders with the extended estimator only "
+                                    "(flags ENV_IS and/or RUSSIAN_ROULETTE): its hits are not the reference's");
     if (ext && d.render_mode != VR_MODE_FULL)
         return fail(VR_ERR_INVALID, "the extended estimator renders RenderMode::Full only");
     if (ext && !(d.firefly_clamp == kInf))
@@ -745,6 +751,7 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         if (creq) {
             r.cmode = creq[i].cmode;
             r.cache_depth = creq[i].cache_depth;
+            r.query_roughness = creq[i].query_roughness;
             r.train_vertices = creq[i].train_vertices;
             r.train_threshold = creq[i].train_threshold;
             r.frame = creq[i].frame;
@@ -1091,6 +1098,7 @@ int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr
     if (f->train_vertices == 0 || f->train_vertices > kNrcMaxTrainVertices)
         return fail(VR_ERR_INVALID, "train_vertices must be in [1, 8]");
     if (f->cache_depth >= desc->max_bounces) return fail(VR_ERR_INVALID, "cache_depth must be below max_bounces");
+    if (f->train_steps > kNrcMaxParts) return fail(VR_ERR_INVALID, "train_steps must be at most 32");
     const uint64_t npx = (uint64_t)desc->width * desc->height;
     if (npx == 0 || desc->samples == 0) return 0;
     if (npx >= (1ull << 32)) return fail(VR_ERR_INVALID, "image too large");
@@ -1109,15 +1117,18 @@ int render_cached(vr_scene* s, vr_cache* c, const vr_render_desc* desc, const vr
     cr.train_vertices = f->train_vertices;
     cr.train_threshold = (uint32_t)std::min<double>(4294967295.0, (double)train_paths / (double)paths * 4294967296.0);
     cr.frame = f->frame;
+    cr.query_roughness = f->query_roughness;
     VR_HIP_CHECK(hipEventRecord(c->ev[0], st));
     vr_request rq{s, *desc, d_rgba};
     rc = render_batch(ctx, &rq, 1, nullptr, 0, st, &cr, c);
     if (rc) return rc;
     VR_HIP_CHECK(hipEventRecord(c->ev[1], st));
-    // one Adam step per mini-batch: the frame's samples split into train_steps slices
+    // one Adam step per mini-batch: the frame's samples, reordered into
+    // train_steps random mini-batches (Morton order inside each), in slices
+    if (f->train_steps && train_paths) rc = nrc_order_samples(c, f->train_steps, f->frame, st);
     for (uint32_t k = 0; k < f->train_steps && train_paths && !rc; ++k)
-        rc = nrc_train_step(c, c->tr.cap, c->tr.count, c->tr.pos, c->tr.dir, c->tr.nrm, c->tr.rough, c->tr.target, true,
-                            st, k, f->train_steps);
+        rc = nrc_train_step(c, c->trs.cap, c->trs.count, c->trs.pos, c->trs.dir, c->trs.nrm, c->trs.rough,
+                            c->trs.target, true, st, k, f->train_steps);
     if (rc) return rc;
     VR_HIP_CHECK(hipEventRecord(c->ev[2], st));
     c->frame_timed = true;

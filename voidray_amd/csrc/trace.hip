@@ -32,11 +32,12 @@ struct TraceArgs {
     uint32_t* fb_list;
     uint32_t* fb_count;
     uint32_t* next;       // partition counters of the dynamic ray fetch (kFetchStride apart)
-    u32x2* spill;         // [kSpillCap][spill_lanes]
+    u32x2* spill;         // [spill_cap][spill_lanes]
     uint32_t spill_lanes;
     const uint32_t* queue;  // optional: item k traces ray queue[k] (else ray k)
     const uint32_t* count;  // optional: device-side item count (else n)
     float tmax;             // any-hit: hits at t >= tmax do not count
+    int spill_cap;          // global spill entries per lane; a deeper stack hands the ray to the exact engine
 };
 
 __device__ __forceinline__ void write_out(const TraceArgs& a, uint32_t i, const HitOut& h) {
@@ -81,13 +82,13 @@ static_assert(kFetchParts <= 64, "one lane clears each partition counter");
 // spatial-split BVH a ray needs only ~20 steps, so refills wait until a
 // good part of the wave is idle (24 of 64 lanes; 40 for the shorter any-hit
 // rays) and draw from partitioned counters (below).
-template <bool ANY>
+template <bool ANY, bool WT>
 __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
     __shared__ u32x2 lds[LDS * kWave];
     const int lane = threadIdx.x;
     const uint32_t glane = blockIdx.x * kWave + lane;
-    const WideStack ws{lds + lane, kWave, a.spill + glane, (int)a.spill_lanes, kSpillCap};
+    const WideStack ws{lds + lane, kWave, a.spill + glane, (int)a.spill_lanes, a.spill_cap};
     const uint32_t kNone = 0xFFFFFFFFu;
     const uint32_t n = a.count ? *a.count : a.n;
     WideState st;
@@ -122,8 +123,8 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                     ray = r;
                     vf3 o = mk3(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2]);
                     vf3 d = mk3(a.d[3 * r], a.d[3 * r + 1], a.d[3 * r + 2]);
-                    wide_init(st, o, d, a.sv.num_qnodes > 0);
-                    if (a.sv.num_analytic) wide_seed_analytic<ANY>(a.sv, st, a.tmin, a.tmax);
+                    wide_init<WT>(st, o, d, a.sv.num_qnodes > 0);
+                    if (a.sv.num_analytic) wide_seed_analytic<ANY, WT>(a.sv, st, a.tmin, a.tmax);
                 }
             }
         }
@@ -158,11 +159,11 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
             if (wide_node<LDS, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
         }
         // Leaf phase
-        if (leaf != kNone && !fallback && wide_leaf<ANY>(a.sv, st, leaf, a.tmin, a.tmax) == kStepDone) done = true;
+        if (leaf != kNone && !fallback && wide_leaf<ANY, WT>(a.sv, st, leaf, a.tmin, a.tmax) == kStepDone) done = true;
         if (live && !done && !fallback && !st.have && st.sp == 0) done = true;
         if (done || fallback) {
             HitOut h;
-            int rc = fallback ? kStepFallback : wide_finish<ANY>(a.sv, st, a.tmin, h);
+            int rc = fallback ? kStepFallback : wide_finish<ANY, WT>(a.sv, st, a.tmin, h);
             if (rc == kStepFallback) {
                 uint32_t slot = atomicAdd(a.fb_count, 1u);
                 a.fb_list[slot] = ray;
@@ -178,15 +179,24 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
 // stack then bounds residency at 6 waves; a 4 KB stack gave 8 waves and ~1%
 // more on short rays but spilled more on long ones).  Any hit keeps the
 // registers it wants (7 waves): held to 64 it spills.
-template <bool ANY>
+template <bool ANY, bool WT = false>
 __global__ void k_trace_persist(TraceArgs a);
 template <>
-__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false>(TraceArgs a) {
-    trace_persist<false>(a);
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, false>(TraceArgs a) {
+    trace_persist<false, false>(a);
 }
 template <>
-__global__ __launch_bounds__(kWave) void k_trace_persist<true>(TraceArgs a) {
-    trace_persist<true>(a);
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, false>(TraceArgs a) {
+    trace_persist<true, false>(a);
+}
+// watertight engine (extended estimator only)
+template <>
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, true>(TraceArgs a) {
+    trace_persist<false, true>(a);
+}
+template <>
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, true>(TraceArgs a) {
+    trace_persist<true, true>(a);
 }
 
 __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
@@ -194,11 +204,12 @@ __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
 }
 
 // Serial exact engine for one ray, with its stacks in the caller's LDS.
+template <bool WT>
 __device__ __forceinline__ void exact_serial(const TraceArgs& a, uint32_t i, int32_t* tstack, int32_t* mstack, int stride) {
     vf3 o = mk3(a.o[3 * i], a.o[3 * i + 1], a.o[3 * i + 2]);
     vf3 d = mk3(a.d[3 * i], a.d[3 * i + 1], a.d[3 * i + 2]);
     HitOut h;
-    trace_exact_stk(a.sv, o, d, a.tmin, h, tstack, mstack, stride);
+    trace_exact_stk<WT>(a.sv, o, d, a.tmin, h, tstack, mstack, stride);
     if (a.any && h.prim != 0xFFFFFFFFu && !(h.t < a.tmax)) h.prim = 0xFFFFFFFFu;
     write_out(a, i, h);
 }
@@ -216,6 +227,7 @@ constexpr int kCoopStack = 6144;
 static_assert(kCoopStack * 2 >= 2 * 64 * kWave, "the serial stacks reuse the cooperative stack");
 enum : uint32_t { kCoopTop = 0u, kCoopMesh = 1u, kCoopTri = 2u };
 
+template <bool WT>
 __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const uint32_t* list, uint32_t* ctr,
                                                             uint32_t* serial_list, uint64_t* accum) {
     __shared__ u32x2 stk[kCoopStack];
@@ -239,6 +251,8 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
         vf3 d = mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]);
         float bt = kInf, bu = 0.f, bv = 0.f;
         uint32_t bprim = 0xFFFFFFFFu;
+        WTRay wt{};
+        if (WT) wt = wt_prepare(d);
         int sp = sv.top_root_valid ? 1 : 0;  // wave-uniform
         if (lane == 0) stk[0] = u32x2{kCoopTop << 30, 0u};
         bool overflow = false;
@@ -284,7 +298,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         }
                     } else {
                         Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
-                        if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
+                        if (WT ? wt_slab_hit(b, o, d, a.tmin, kInf) : ref_slab_hit(b, o, d, a.tmin, kInf)) {
                             e0 = u32x2{(kCoopTop << 30) | (uint32_t)nd.right, 0u};
                             e1 = u32x2{(kCoopTop << 30) | (idx + 1), 0u};
                             np = 2;
@@ -300,7 +314,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                             tri = nd.obj;
                         } else {
                             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
-                            if (ref_slab_hit(b, o, d, a.tmin, kInf)) {
+                            if (WT ? wt_slab_hit(b, o, d, a.tmin, kInf) : ref_slab_hit(b, o, d, a.tmin, kInf)) {
                                 e0 = u32x2{(kCoopMesh << 30) | (uint32_t)nd.right, surf};
                                 e1 = u32x2{(kCoopMesh << 30) | (idx + 1), surf};
                                 np = 2;
@@ -316,7 +330,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
                         vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
                         MTResult r;
-                        if (mt_hit(o, d, p0, p1, p2, a.tmin, r)) {
+                        if (WT ? wt_hit(wt, o, p0, p1, p2, a.tmin, r) : mt_hit(o, d, p0, p1, p2, a.tmin, r)) {
                             const uint32_t prim = si.prim_base + (uint32_t)tri;
                             if (r.t < bt || (r.t == bt && prim < bprim)) {
                                 bt = r.t;
@@ -391,7 +405,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
     if (ns) {  // the cooperative stack is free now: 64 + 64 stack entries per lane
         int32_t* lds32 = (int32_t*)stk;
         for (uint32_t k = lane; k < ns; k += kWave)
-            exact_serial(a, load_agent(&serial_list[k]), lds32 + lane, lds32 + 64 * kWave + lane, kWave);
+            exact_serial<WT>(a, load_agent(&serial_list[k]), lds32 + lane, lds32 + 64 * kWave + lane, kWave);
     }
     if (lane == 0) {
         uint32_t fb = list ? load_agent(&ctr[kCtrFallbacks]) : 0u;
@@ -405,29 +419,29 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
     if (lane < kFetchParts) ctr[kCtrFetch + lane * kFetchStride] = 0u;
 }
 
-static int persist_grid(vr_scene* s, bool any, uint32_t& blocks) {
-    if (s->persist_blocks[any ? 1 : 0] == 0) {
+static int persist_grid(vr_scene* s, bool any, bool wt, uint32_t& blocks) {
+    const int k = (any ? 1 : 0) + (wt ? 2 : 0);
+    if (s->persist_blocks[k] == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        if (any)
-            VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persist<true>, kWave, 0));
-        else
-            VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persist<false>, kWave, 0));
-        s->persist_blocks[any ? 1 : 0] = (uint32_t)std::max(1, cus * std::max(1, per_cu));
+        void (*const kern[4])(TraceArgs) = {k_trace_persist<false, false>, k_trace_persist<true, false>,
+                                             k_trace_persist<false, true>, k_trace_persist<true, true>};
+        VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern[k], kWave, 0));
+        s->persist_blocks[k] = (uint32_t)std::max(1, cus * std::max(1, per_cu));
     }
-    blocks = s->persist_blocks[any ? 1 : 0];
+    blocks = s->persist_blocks[k];
     return 0;
 }
 
 int ensure_trace_scratch(vr_scene* s, uint32_t n) {
     if (!s->spill) {
-        uint32_t b0 = 0, b1 = 0;
-        int rc = persist_grid(s, false, b0);
-        if (rc) return rc;
-        rc = persist_grid(s, true, b1);
-        if (rc) return rc;
-        s->spill_lanes = std::max(b0, b1) * kWave;
+        uint32_t b[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) {
+            int rc = persist_grid(s, k & 1, k & 2, b[k]);
+            if (rc) return rc;
+        }
+        s->spill_lanes = std::max(std::max(b[0], b[1]), std::max(b[2], b[3])) * kWave;
         size_t bytes = (size_t)kSpillCap * s->spill_lanes * sizeof(u32x2);
         if (hipMalloc((void**)&s->spill, bytes) != hipSuccess) return fail(VR_ERR_OOM, "spill stack allocation failed");
     }
@@ -450,34 +464,64 @@ int ensure_trace_scratch(vr_scene* s, uint32_t n) {
     return 0;
 }
 
+// Device scratch of one trace launch: the counter block (kCtrWords), the
+// wide engine's fallback list and the exact engine's overflow list (one entry
+// per ray each), and the per-lane spill stacks of the persistent kernels.
+struct TraceScratch {
+    uint32_t* counters;
+    uint32_t* fb_list;
+    uint32_t* serial_list;
+    u32x2* spill;
+    uint32_t spill_lanes;   // lanes the spill stacks cover (the grid may use fewer)
+    int spill_cap;
+};
+
+static int launch_trace_with(vr_scene* s, const TraceJob& j, const TraceScratch& sc, hipStream_t stream) {
+    TraceArgs a{s->view, j.o, j.d, j.n_max, j.tmin, j.any ? 1 : 0, j.prim, j.t, j.u, j.v, j.hit,
+                sc.fb_list, sc.counters, sc.counters + kCtrFetch, sc.spill, sc.spill_lanes, j.queue, j.count, j.tmax,
+                sc.spill_cap};
+    const bool wt = j.engine == VR_ENGINE_WATERTIGHT;
+    uint64_t* accum = j.engine != VR_ENGINE_REFERENCE ? j.fallback_accum : nullptr;
+    if (j.engine != VR_ENGINE_REFERENCE) {
+        uint32_t blocks = 0;
+        int rc = persist_grid(s, j.any, wt, blocks);
+        if (rc) return rc;
+        blocks = std::min<uint32_t>(blocks, sc.spill_lanes / kWave);
+        blocks = std::min<uint32_t>(blocks, (j.n_max + kWave - 1) / kWave);
+        void (*persist)(TraceArgs) = j.any ? (wt ? k_trace_persist<true, true> : k_trace_persist<true, false>)
+                                           : (wt ? k_trace_persist<false, true> : k_trace_persist<false, false>);
+        void (*exact)(TraceArgs, const uint32_t*, uint32_t*, uint32_t*, uint64_t*) =
+            wt ? k_trace_exact_wave<true> : k_trace_exact_wave<false>;
+        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[0], stream));
+        hipLaunchKernelGGL(persist, dim3(blocks), dim3(kWave), 0, stream, a);
+        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[1], stream));
+        // fallbacks: one wave per ray; the grid reads the device-side count (no host sync)
+        hipLaunchKernelGGL(exact, dim3(1024), dim3(kWave), 0, stream, a, (const uint32_t*)sc.fb_list, sc.counters,
+                           sc.serial_list, accum);
+    } else {
+        uint32_t blocks = std::min<uint32_t>(j.n_max, 65535u);
+        void (*exact)(TraceArgs, const uint32_t*, uint32_t*, uint32_t*, uint64_t*) = k_trace_exact_wave<false>;
+        hipLaunchKernelGGL(exact, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr, sc.counters,
+                           sc.serial_list, accum);
+    }
+    VR_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
     if (j.n_max == 0) return 0;
     int rc = ensure_trace_scratch(s, j.n_max);
     if (rc) return rc;
-    TraceArgs a{s->view, j.o, j.d, j.n_max, j.tmin, j.any ? 1 : 0, j.prim, j.t, j.u, j.v, j.hit,
-                s->fallback_list, s->counters, s->counters + kCtrFetch, s->spill, s->spill_lanes, j.queue, j.count, j.tmax};
-    uint32_t* serial_list = s->fallback_list + s->fallback_cap / 2;
-    uint64_t* accum = j.engine == VR_ENGINE_WIDE ? j.fallback_accum : nullptr;
-    if (j.engine == VR_ENGINE_WIDE) {
-        uint32_t blocks = 0;
-        rc = persist_grid(s, j.any, blocks);
-        if (rc) return rc;
-        blocks = std::min<uint32_t>(blocks, (j.n_max + kWave - 1) / kWave);
-        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[0], stream));
-        if (j.any)
-            hipLaunchKernelGGL(k_trace_persist<true>, dim3(blocks), dim3(kWave), 0, stream, a);
-        else
-            hipLaunchKernelGGL(k_trace_persist<false>, dim3(blocks), dim3(kWave), 0, stream, a);
-        if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[1], stream));
-        // fallbacks: one wave per ray; the grid reads the device-side count (no host sync)
-        hipLaunchKernelGGL(k_trace_exact_wave, dim3(1024), dim3(kWave), 0, stream, a,
-                           (const uint32_t*)s->fallback_list, s->counters, serial_list, accum);
-    } else {
-        uint32_t blocks = std::min<uint32_t>(j.n_max, 65535u);
-        hipLaunchKernelGGL(k_trace_exact_wave, dim3(blocks), dim3(kWave), 0, stream, a, (const uint32_t*)nullptr,
-                           s->counters, serial_list, accum);
-    }
-    VR_HIP_CHECK(hipGetLastError());
+    TraceScratch sc{s->counters, s->fallback_list, s->fallback_list + s->fallback_cap / 2, s->spill, s->spill_lanes,
+                    kSpillCap};
+    return launch_trace_with(s, j, sc, stream);
+}
+
+static int check_trace_desc(const vr_trace_desc* desc, const float* d_o, const float* d_d) {
+    if (!d_o || !d_d) return fail(VR_ERR_INVALID, "null ray buffers");
+    if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
+    if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE && desc->engine != VR_ENGINE_WATERTIGHT)
+        return fail(VR_ERR_INVALID, "bad engine");
     return 0;
 }
 
@@ -485,10 +529,8 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
                  float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream) {
     uint32_t n = desc->num_rays;
     if (n == 0) return 0;
-    if (!d_o || !d_d) return fail(VR_ERR_INVALID, "null ray buffers");
-    if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
-    if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE)
-        return fail(VR_ERR_INVALID, "bad engine");
+    int rc = check_trace_desc(desc, d_o, d_d);
+    if (rc) return rc;
     TraceJob j;
     j.o = d_o;
     j.d = d_d;
@@ -506,9 +548,9 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
         VR_HIP_CHECK(hipEventCreate(&s->trace_ev[0]));
         VR_HIP_CHECK(hipEventCreate(&s->trace_ev[1]));
     }
-    j.timed = desc->engine == VR_ENGINE_WIDE;
+    j.timed = desc->engine != VR_ENGINE_REFERENCE;
     s->trace_timed = j.timed;
-    int rc = launch_trace(s, j, stream);
+    rc = launch_trace(s, j, stream);
     if (rc) return rc;
     VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters + kCtrLastFallbacks, 2 * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, stream));

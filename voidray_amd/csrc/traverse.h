@@ -29,6 +29,13 @@
 // (mesh.rs:269-279; the brute-force loop of mesh.rs:125-138 keeps the earlier
 // triangle, the same rule).  Prim ids are numbered surface by surface in
 // triangle order, so both engines keep the lexicographic minimum of (t, prim).
+//
+// Watertight mode (template flag WT, engine VR_ENGINE_WATERTIGHT): the same
+// traversal with the watertight ray/triangle test of vrmath.h in place of
+// Moller-Trumbore and no reference semantics: every candidate the
+// conservative boxes reach counts (no ancestor confirmation), and the exact
+// fallback walks the median-split trees with slab tests that never reject a
+// ray crossing the box.  For the extended estimator only.
 #pragma once
 #include "internal.h"
 
@@ -151,7 +158,9 @@ VR_HD bool top_path_pass(const SceneView& sv, uint32_t surf, vf3 o, vf3 d, float
 
 // Closest analytic surface the reference reaches (lexicographic min of (t,
 // prim), like the triangles); with `first_below` set, the first one found
-// with t < first_below.  Returns its index or -1.
+// with t < first_below.  Returns its index or -1.  WT: every analytic
+// surface counts (no top-level reach test).
+template <bool WT = false>
 VR_HD int analytic_best(const SceneView& sv, vf3 o, vf3 d, float tmin, float first_below, float& bt, uint32_t& bprim,
                         float& bu, float& bv) {
     int best = -1;
@@ -160,7 +169,7 @@ VR_HD int analytic_best(const SceneView& sv, vf3 o, vf3 d, float tmin, float fir
         float t, u, v;
         if (!analytic_hit(a, o, d, tmin, t, u, v)) continue;
         if (best >= 0 && !(t < bt || (t == bt && a.prim < bprim))) continue;
-        if (!top_path_pass(sv, a.surface, o, d, tmin)) continue;
+        if (!WT && !top_path_pass(sv, a.surface, o, d, tmin)) continue;
         best = (int)k;
         bt = t;
         bprim = a.prim;
@@ -181,6 +190,7 @@ constexpr uint32_t kAnalyticSlot = 0x40000000u;  // best_slot of an analytic win
 // lies beyond the current best hit is skipped without touching its node.
 struct WideState {
     vf3 o, d;
+    WTRay wt;  // watertight mode: the ray's shear (unused otherwise)
     float idx, idy, idz;
     float tbest, bu, bv;
     uint32_t best_prim, best_slot;
@@ -198,10 +208,12 @@ struct WideStack {
 
 enum : int { kStepContinue = 0, kStepDone = 1, kStepFallback = 2 };
 
+template <bool WT = false>
 VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     const float kTiny = 1e-20f;
     st.o = o;
     st.d = d;
+    if (WT) st.wt = wt_prepare(d);
     st.idx = 1.0f / (fabsf(d.x) > kTiny ? d.x : copysignf(kTiny, d.x));
     st.idy = 1.0f / (fabsf(d.y) > kTiny ? d.y : copysignf(kTiny, d.y));
     st.idz = 1.0f / (fabsf(d.z) > kTiny ? d.z : copysignf(kTiny, d.z));
@@ -221,11 +233,11 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
 // up front, and the best one seeds the closest hit (so the triangle walk culls
 // against it and ties resolve by prim id as usual).  An any-hit ray that hits
 // one below tmax_any is finished.
-template <bool ANY>
+template <bool ANY, bool WT = false>
 VR_HD void wide_seed_analytic(const SceneView& sv, WideState& st, float tmin, float tmax_any) {
     float t = kInf, u = 0.f, v = 0.f;
     uint32_t prim = 0xFFFFFFFFu;
-    const int k = analytic_best(sv, st.o, st.d, tmin, ANY ? tmax_any : -kInf, t, prim, u, v);
+    const int k = analytic_best<WT>(sv, st.o, st.d, tmin, ANY ? tmax_any : -kInf, t, prim, u, v);
     if (k < 0) return;
     if (ANY) {
         if (!(t < tmax_any)) return;
@@ -283,7 +295,7 @@ VR_HD bool wide_pop_next(WideState& st, const WideStack& ws) {
 
 // Tests the triangles of leaf `code`.  Returns kStepContinue, or kStepDone
 // when an any-hit query has its confirmed hit.
-template <bool ANY>
+template <bool ANY, bool WT = false>
 VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmin, float tmax_any) {
     const vf3 o = st.o, d = st.d;
     uint32_t first = code & 0x07FFFFFFu;
@@ -294,11 +306,11 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
         vf3 p0, p1, p2;
         load_tri_geom(sv.tris, slot, p0, p1, p2);
         MTResult r;
-        if (!mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
+        if (WT ? !wt_hit(st.wt, o, p0, p1, p2, tmin, r) : !mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
         if (ANY) {
             if (r.t < tmax_any) {
-                if (confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, slot, 11),
-                                   tri_word(sv.tris, slot, 9))) {
+                if (WT || confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, slot, 11),
+                                         tri_word(sv.tris, slot, 9))) {
                     st.found = true;
                     st.tbest = r.t;
                     st.best_slot = slot;
@@ -391,13 +403,13 @@ VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uin
 
 // One traversal step (one node or one leaf).  Returns kStepContinue,
 // kStepDone (result ready for wide_finish) or kStepFallback.
-template <int LDS_STACK, bool ANY>
+template <int LDS_STACK, bool ANY, bool WT = false>
 VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
     if (!st.have && !wide_pop_next<LDS_STACK, ANY>(st, ws)) return kStepDone;
     st.have = false;
     const uint32_t cur = st.cur;
     if (cur & kLeafBit) {
-        int rc = wide_leaf<ANY>(sv, st, cur, tmin, tmax_any);
+        int rc = wide_leaf<ANY, WT>(sv, st, cur, tmin, tmax_any);
         return rc == kStepDone ? kStepDone : kStepContinue;
     }
     return wide_node<LDS_STACK, ANY>(sv, st, ws, cur, tmin, tmax_any);
@@ -405,7 +417,7 @@ VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, flo
 
 // Result of a finished query (after kStepDone).  Returns kStepDone or
 // kStepFallback (winner not confirmed / any-hit candidates unconfirmed).
-template <bool ANY>
+template <bool ANY, bool WT = false>
 VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitOut& out) {
     out.t = 0.f;
     out.u = 0.f;
@@ -432,8 +444,8 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
     }
     vf3 p0, p1, p2;
     load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
-    if (!confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
-                        tri_word(sv.tris, st.best_slot, 9)))
+    if (!WT && !confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
+                               tri_word(sv.tris, st.best_slot, 9)))
         return kStepFallback;
     out.t = st.tbest;
     out.u = st.bu;
@@ -443,30 +455,34 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
 }
 
 // Whole query for one ray (used by the host-side checks and simple kernels).
-template <int LDS_STACK, bool ANY>
+template <int LDS_STACK, bool ANY, bool WT = false>
 VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, const WideStack& ws,
                      HitOut& out) {
     WideState st;
-    wide_init(st, o, d, sv.num_qnodes > 0);
-    if (sv.num_analytic) wide_seed_analytic<ANY>(sv, st, tmin, tmax_any);
+    wide_init<WT>(st, o, d, sv.num_qnodes > 0);
+    if (sv.num_analytic) wide_seed_analytic<ANY, WT>(sv, st, tmin, tmax_any);
     int rc;
-    while ((rc = wide_step<LDS_STACK, ANY>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
+    while ((rc = wide_step<LDS_STACK, ANY, WT>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
     }
     if (rc == kStepFallback) {
         out.prim = 0xFFFFFFFFu;
         return kStepFallback;
     }
-    return wide_finish<ANY>(sv, st, tmin, out);
+    return wide_finish<ANY, WT>(sv, st, tmin, out);
 }
 
 // Exact reference engine for one ray (closest hit).  The two traversal
 // stacks (64 entries each, element k at [k * stride]) are the caller's: local
-// arrays, or a lane's slice of LDS.
+// arrays, or a lane's slice of LDS.  WT: the watertight engine's fallback
+// (conservative slab tests, watertight triangle test).
+template <bool WT = false>
 __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out,
                                                 int32_t* tstack, int32_t* mstack, int stride) {
     out.prim = 0xFFFFFFFFu;
     out.t = out.u = out.v = 0.f;
     if (!sv.top_root_valid) return;
+    WTRay wt{};
+    if (WT) wt = wt_prepare(d);
     bool have = false;
     float bt = 0.f;
     int tsp = 0;
@@ -476,7 +492,7 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
         RefNode tn = sv.top_nodes[ni];
         if (tn.obj < 0) {
             Box3 b{mk3(tn.lo[0], tn.lo[1], tn.lo[2]), mk3(tn.hi[0], tn.hi[1], tn.hi[2])};
-            if (ref_slab_hit(b, o, d, tmin, kInf)) {
+            if (WT ? wt_slab_hit(b, o, d, tmin, kInf) : ref_slab_hit(b, o, d, tmin, kInf)) {
                 tstack[tsp++ * stride] = tn.right;
                 tstack[tsp++ * stride] = ni + 1;
             }
@@ -505,7 +521,7 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
             vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
             vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
             MTResult r;
-            if (!mt_hit(o, d, p0, p1, p2, tmin, r)) return;
+            if (WT ? !wt_hit(wt, o, p0, p1, p2, tmin, r) : !mt_hit(o, d, p0, p1, p2, tmin, r)) return;
             const uint32_t prim = si.prim_base + tri;
             if (!have || r.t < bt || (r.t == bt && prim < out.prim)) {
                 have = true;
@@ -530,16 +546,17 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
                 continue;
             }
             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
-            if (!ref_slab_hit(b, o, d, tmin, kInf)) continue;
+            if (!(WT ? wt_slab_hit(b, o, d, tmin, kInf) : ref_slab_hit(b, o, d, tmin, kInf))) continue;
             mstack[msp++ * stride] = nd.right;
             mstack[msp++ * stride] = mj + 1;
         }
     }
 }
 
+template <bool WT = false>
 __host__ __device__ inline void trace_exact(const SceneView& sv, vf3 o, vf3 d, float tmin, HitOut& out) {
     int32_t tstack[64], mstack[64];
-    trace_exact_stk(sv, o, d, tmin, out, tstack, mstack, 1);
+    trace_exact_stk<WT>(sv, o, d, tmin, out, tstack, mstack, 1);
 }
 
 }  // namespace vr

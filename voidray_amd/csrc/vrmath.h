@@ -171,3 +171,84 @@ VR_HD bool mt_hit(vf3 o, vf3 d, vf3 p0, vf3 p1, vf3 p2, float tmin, MTResult& r)
     r.v = v;
     return true;
 }
+
+// ---- watertight ray/triangle test (Woop, Benthin & Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013) ----------------------------
+// The ray is sheared so that it runs along +z of a permuted frame; each edge
+// function depends only on the ray and the edge's two vertices, and is
+// computed as fl(fl(a*b) - fl(c*d)), which negates exactly when the edge is
+// walked the other way round (no contraction: -ffp-contract=off).  Two
+// triangles sharing an edge therefore see the same value with opposite signs,
+// and a ray through the edge (or a shared vertex) is inside at least one of
+// them: no ray slips through a closed mesh.  An edge value of exactly zero
+// counts as inside on both sides.  No counterpart in the reference, which
+// uses Moller-Trumbore (core/mesh.rs:144-188); used by the extended estimator
+// only.
+struct WTRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+VR_HD WTRay wt_prepare(vf3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    WTRay w;
+    w.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    w.kx = w.kz == 2 ? 0 : w.kz + 1;
+    w.ky = w.kx == 2 ? 0 : w.kx + 1;
+    const float dz = comp(d, w.kz);
+    if (dz < 0.0f) {  // keep the winding of the permuted frame
+        const int t = w.kx;
+        w.kx = w.ky;
+        w.ky = t;
+    }
+    w.sx = comp(d, w.kx) / dz;
+    w.sy = comp(d, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+// Closest-hit candidate with t > tmin; u, v = barycentric weights of p1, p2
+// (the same roles as Moller-Trumbore's u, v).
+VR_HD bool wt_hit(const WTRay& w, vf3 o, vf3 p0, vf3 p1, vf3 p2, float tmin, MTResult& r) {
+    const vf3 A = p0 - o, B = p1 - o, C = p2 - o;
+    const float Az = comp(A, w.kz), Bz = comp(B, w.kz), Cz = comp(C, w.kz);
+    const float Ax = comp(A, w.kx) - w.sx * Az, Ay = comp(A, w.ky) - w.sy * Az;
+    const float Bx = comp(B, w.kx) - w.sx * Bz, By = comp(B, w.ky) - w.sy * Bz;
+    const float Cx = comp(C, w.kx) - w.sx * Cz, Cy = comp(C, w.ky) - w.sy * Cz;
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (w.sz * Az) + V * (w.sz * Bz) + W * (w.sz * Cz);
+    const float inv = 1.0f / det;
+    const float t = T * inv;
+    if (!(t > tmin)) return false;
+    r.t = t;
+    r.u = V * inv;
+    r.v = W * inv;
+    return true;
+}
+
+// Slab test that never rejects a ray crossing the box, for the watertight
+// engine's exact fallback: the box grows by 2^-20 of its coordinates' size
+// and an empty interval is tolerated up to 2^-17 relative.
+VR_HD bool wt_slab_hit(const Box3& b, vf3 o, vf3 d, float t_min, float t_max) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float lo = comp(b.lo, a), hi = comp(b.hi, a);
+        const float pad = (fabsf(lo) + fabsf(hi)) * (1.0f / 1048576.0f) + 1e-30f;
+        const float inv_d = 1.0f / comp(d, a);
+        float t0 = (lo - pad - comp(o, a)) * inv_d;
+        float t1 = (hi + pad - comp(o, a)) * inv_d;
+        if (inv_d < 0.0f) {
+            const float tmp = t0;
+            t0 = t1;
+            t1 = tmp;
+        }
+        if (t0 > t_min) t_min = t0;
+        if (t1 < t_max) t_max = t1;
+    }
+    return t_min <= t_max * (1.0f + 1.0f / 131072.0f) || !(t_min == t_min) || !(t_max == t_max);
+}

@@ -218,8 +218,8 @@ class RadianceCache:
         N.check(N.lib().vr_cache_train_async(self.handle, n, d_pos, d_dir, d_nrm, d_rough, d_target, stream))
 
     def render_frame(self, accel, d_rgba, desc, train_paths=1 << 14, train_vertices=4, cache_depth=2,
-                     train_steps=1, frame=0, stream=None):
-        f = N.vr_cache_frame(train_paths, train_vertices, cache_depth, train_steps, frame, 0)
+                     train_steps=1, frame=0, stream=None, query_roughness=0.0):
+        f = N.vr_cache_frame(train_paths, train_vertices, cache_depth, train_steps, frame, query_roughness)
         N.check(N.lib().vr_render_cached_async(accel.handle, self.handle, C.byref(desc), C.byref(f), d_rgba,
                                                stream))
 
