@@ -238,6 +238,23 @@ int vr_trace_device_async(vr_scene* scene, const vr_trace_desc* desc,
                           const float* d_origins, const float* d_directions,
                           uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
                           uint8_t* d_hit, void* stream);
+/* vr_trace_device_async without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The kernels' scratch (counter block, fallback and overflow lists, traversal
+ * spill stacks: VR_NOALLOC_SCRATCH_BYTES) is one static device array per GPU,
+ * set up by vr_create on that GPU and held until the process ends.  At most
+ * VR_NOALLOC_MAX_RAYS rays per call (more: VR_ERR_INVALID, nothing enqueued).
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (other streams, threads or contexts) race: enqueue them on one stream.
+ * Results equal vr_trace_device_async's; vr_trace_last_fallbacks and
+ * vr_trace_last_timing do not see these calls. */
+#define VR_NOALLOC_MAX_RAYS 1048576u
+#define VR_NOALLOC_SCRATCH_BYTES 75501568u /* 4 KiB + 2 x 4 MiB + 64 MiB */
+int vr_trace_device_noalloc_async(vr_scene* scene, const vr_trace_desc* desc,
+                                  const float* d_origins, const float* d_directions,
+                                  uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
+                                  uint8_t* d_hit, void* stream);
 /* The same queries answered on the host, without a GPU: the scene is built
  * into host memory exactly as vr_scene_upload builds it, and the rays run
  * through the same traversal routines the kernels use (wide engine with its

@@ -184,6 +184,8 @@ pub struct vr_scene_stats {
 }
 
 // ---- ray queries (SceneAcceleration::hit, core/scene.rs:182-185) ------------
+pub const VR_NOALLOC_MAX_RAYS: u32 = 1_048_576;
+pub const VR_NOALLOC_SCRATCH_BYTES: u32 = 75_501_568;
 pub const VR_TRACE_CLOSEST: i32 = 0;
 pub const VR_TRACE_ANY: i32 = 1;
 pub const VR_ENGINE_WIDE: i32 = 0;
@@ -368,6 +370,12 @@ extern "C" {
     pub fn vr_trace_device_async(scene: *mut vr_scene, desc: *const vr_trace_desc, d_origins: *const f32,
                                  d_directions: *const f32, d_prim_id: *mut u32, d_t: *mut f32, d_u: *mut f32,
                                  d_v: *mut f32, d_hit: *mut u8, stream: *mut c_void) -> c_int;
+    // no allocation: scratch is one static device array per GPU (callers under
+    // graph capture or in serving loops that cannot hipMalloc)
+    pub fn vr_trace_device_noalloc_async(scene: *mut vr_scene, desc: *const vr_trace_desc,
+                                         d_origins: *const f32, d_directions: *const f32, d_prim_id: *mut u32,
+                                         d_t: *mut f32, d_u: *mut f32, d_v: *mut f32, d_hit: *mut u8,
+                                         stream: *mut c_void) -> c_int;
     pub fn vr_trace_host(scene: *const vr_scene_desc, desc: *const vr_trace_desc, origins: *const f32,
                          directions: *const f32, prim_id: *mut u32, t: *mut f32, u: *mut f32, v: *mut f32,
                          hit: *mut u8, stats: *mut vr_scene_stats, fallbacks: *mut u32) -> c_int;

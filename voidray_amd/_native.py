@@ -20,6 +20,8 @@ VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
 VR_SURFACE_ANALYTIC = 0x80000000
 VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
 VR_ENGINE_WIDE, VR_ENGINE_REFERENCE, VR_ENGINE_WATERTIGHT = 0, 1, 2
+VR_NOALLOC_MAX_RAYS = 1048576
+VR_NOALLOC_SCRATCH_BYTES = 75501568
 VR_MODE_FULL, VR_MODE_NORMAL = 0, 1
 VR_RENDER_ROWS_BY_HEIGHT = 1
 VR_RENDER_ENV_IS = 2
@@ -165,6 +167,7 @@ SIGNATURES = {
     "vr_scene_get_stats": (C.c_int, [_P, C.POINTER(vr_scene_stats)]),
     "vr_trace": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P]),
     "vr_trace_device_async": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vr_trace_device_noalloc_async": (C.c_int, [_P, C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P, _P]),
     "vr_trace_host": (C.c_int, [C.POINTER(vr_scene_desc), C.POINTER(vr_trace_desc), _P, _P, _P, _P, _P, _P, _P,
                                 C.POINTER(vr_scene_stats), C.POINTER(C.c_uint32)]),
     "vr_trace_last_fallbacks": (C.c_int, [_P, C.POINTER(C.c_uint32)]),

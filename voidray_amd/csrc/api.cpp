@@ -69,6 +69,11 @@ int vr_create(int device, vr_ctx** out) {
             delete c;
             return fail(VR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         }
+        // set-up of the no-allocation trace scratch on this device (a failure
+        // here is reported again by vr_trace_device_noalloc_async)
+        void* scratch_ptr = nullptr;
+        (void)noalloc_scratch(device, &scratch_ptr);
+        set_error("");
         *out = c;
         return 0;
     });
@@ -123,6 +128,16 @@ int vr_trace_device_async(vr_scene* s, const vr_trace_desc* desc, const float* d
         if (!s || !desc) return fail(VR_ERR_INVALID, "null argument");
         hipStream_t st = stream ? (hipStream_t)stream : s->ctx->stream;
         return trace_device(s, desc, d_o, d_d, d_prim, d_t, d_u, d_v, d_hit, st);
+    });
+}
+
+int vr_trace_device_noalloc_async(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d,
+                                  uint32_t* d_prim, float* d_t, float* d_u, float* d_v, uint8_t* d_hit,
+                                  void* stream) {
+    return guarded([&] {
+        if (!s || !desc) return fail(VR_ERR_INVALID, "null argument");
+        hipStream_t st = stream ? (hipStream_t)stream : s->ctx->stream;
+        return trace_device_noalloc(s, desc, d_o, d_d, d_prim, d_t, d_u, d_v, d_hit, st);
     });
 }
 

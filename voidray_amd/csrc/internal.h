@@ -223,4 +223,7 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream);
 int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d, uint32_t* d_prim,
                  float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream);
 int ensure_trace_scratch(vr_scene* s, uint32_t n);
+int noalloc_scratch(int device, void** out);
+int trace_device_noalloc(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d,
+                         uint32_t* d_prim, float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream);
 }  // namespace vr

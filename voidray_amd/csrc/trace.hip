@@ -1,5 +1,8 @@
 // Ray-query kernels: the batch form of SceneAcceleration::hit
 // (voidray_renderer/src/core/scene.rs:182-185) over rays resident in HBM.
+#include <mutex>
+#include <string>
+
 #include "internal.h"
 #include "traverse.h"
 
@@ -555,6 +558,100 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     VR_HIP_CHECK(hipMemcpyAsync(s->host_counters, s->counters + kCtrLastFallbacks, 2 * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, stream));
     return 0;
+}
+
+// ---- ray queries without allocation --------------------------------------
+// vr_trace_device_noalloc_async keeps all of its device scratch in one
+// file-scope array per GPU, laid out as
+//   [counter block: kCtrWords u32][fallback list: kNoAllocMaxRays u32]
+//   [overflow list: kNoAllocMaxRays u32][spill stacks: kNoAllocSpill x kNoAllocLanes x 8 B]
+// The persistent grid is capped at kNoAllocMaxBlocks one-wave blocks (32 per
+// CU on 256 CUs: every resident wave), and a lane's traversal stack may spill
+// kNoAllocSpill entries past its LDS ring; a deeper stack sends the ray to the
+// exact engine, which gives the same answer.
+constexpr uint32_t kNoAllocMaxRays = VR_NOALLOC_MAX_RAYS;
+constexpr uint32_t kNoAllocMaxBlocks = 8192;
+constexpr uint32_t kNoAllocLanes = kNoAllocMaxBlocks * kWave;
+constexpr int kNoAllocSpill = 16;
+constexpr size_t kNoAllocCtrBytes = (size_t)kCtrWords * sizeof(uint32_t);
+constexpr size_t kNoAllocListBytes = (size_t)kNoAllocMaxRays * sizeof(uint32_t);
+constexpr size_t kNoAllocSpillBytes = (size_t)kNoAllocSpill * kNoAllocLanes * sizeof(u32x2);
+constexpr size_t kNoAllocBytes = kNoAllocCtrBytes + 2 * kNoAllocListBytes + kNoAllocSpillBytes;
+static_assert(kNoAllocBytes == VR_NOALLOC_SCRATCH_BYTES, "scratch size differs from the header's");
+static_assert(kNoAllocCtrBytes % 256 == 0 && kNoAllocListBytes % 256 == 0, "scratch parts stay 256-byte aligned");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+__global__ void k_clear_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Device address of `scratch` on `device`, looked up once per device (with
+// that device current) and kept: in host code the array's name is only a
+// host-side stand-in, never a device address.
+int noalloc_scratch(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(VR_ERR_INVALID, "device index out of range for the trace scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        VR_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != device) VR_HIP_CHECK(hipSetDevice(device));
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && p) {
+            // the counter block must start at zero; every launch leaves it so
+            hipLaunchKernelGGL(k_clear_words, dim3(4), dim3(256), 0, nullptr, (uint32_t*)p, (uint32_t)kCtrWords);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(VR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
+int trace_device_noalloc(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const float* d_d,
+                         uint32_t* d_prim, float* d_t, float* d_u, float* d_v, uint8_t* d_hit, hipStream_t stream) {
+    const uint32_t n = desc->num_rays;
+    if (n > kNoAllocMaxRays)
+        return fail(VR_ERR_INVALID, "vr_trace_device_noalloc_async traces at most VR_NOALLOC_MAX_RAYS (1,048,576) "
+                                    "rays per call; split the batch");
+    if (n == 0) return 0;
+    int rc = check_trace_desc(desc, d_o, d_d);
+    if (rc) return rc;
+    void* base = nullptr;
+    rc = noalloc_scratch(s->ctx->device, &base);
+    if (rc) return rc;
+    uint8_t* b = (uint8_t*)base;
+    TraceScratch sc{(uint32_t*)b, (uint32_t*)(b + kNoAllocCtrBytes),
+                    (uint32_t*)(b + kNoAllocCtrBytes + kNoAllocListBytes),
+                    (u32x2*)(b + kNoAllocCtrBytes + 2 * kNoAllocListBytes), kNoAllocLanes, kNoAllocSpill};
+    // whatever an earlier call on this GPU left, this call starts from zero
+    hipLaunchKernelGGL(k_clear_words, dim3(4), dim3(256), 0, stream, sc.counters, (uint32_t)kCtrWords);
+    TraceJob j;
+    j.o = d_o;
+    j.d = d_d;
+    j.n_max = n;
+    j.tmin = desc->t_min;
+    j.tmax = kInf;
+    j.any = desc->mode == VR_TRACE_ANY;
+    j.engine = desc->engine;
+    j.prim = d_prim;
+    j.t = d_t;
+    j.u = d_u;
+    j.v = d_v;
+    j.hit = d_hit;
+    j.timed = false;
+    return launch_trace_with(s, j, sc, stream);
 }
 
 }  // namespace vr
