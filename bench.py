@@ -406,6 +406,25 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
         leg["precision_vs_fp64_oracle"] = _precision(g1, ref64, rows=slice(0, None, 30))
         leg["precision_note"] = "1 spp, seed 5, same random streams; f32 rows 0,3,6,...; FP64 rows 0,30,60,..."
     out["terrain_1080p_64spp"] = leg
+    # the same frame with the 200 spheres as instances of one unit sphere
+    # (one shared object-space BVH): same random draws, same streams
+    t0 = time.time()
+    c3i = W.terrain_spheres(instanced=True)
+    a3i = c3i.build_acceleration(ctx)
+    build3i = time.time() - t0
+    imi = np.zeros((1080, 1920, 4), np.float32)
+    a3i.render(imi, make_render_desc(1920, 1080, 1, c3_spp, s3, seed=99, flags=ext, rr_depth=4))  # warm-up
+    imi[:] = 0
+    st = a3i.render(imi, make_render_desc(1920, 1080, c3_spp, c3_spp, s3, seed=1, flags=ext, rr_depth=4))
+    x, r = imi[..., :3].astype(np.float64), img3[..., :3].astype(np.float64)
+    out["terrain_1080p_64spp_instanced"] = {
+        "ms": round(st["kernel_ms"], 2), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
+        "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
+        "exact_engine_rays": st["fallback_rays"], "instances": len(c3i.instances), "scene_build_s": round(build3i, 2),
+        "device_bytes": a3i.stats()["device_bytes"], "device_bytes_flattened": a3.stats()["device_bytes"],
+        "relmse_vs_flattened_same_streams": float(np.mean((x - r) ** 2 / (r * r + 0.01))),
+        "roofline": _render_roofline(st, env_cdf=True)}
+    a3i.release()
     # the same frame under the reference estimator (no RR, no env IS, clamp 3)
     sr = RenderSettings(total_samples=c3_spp, max_bounces=16)
     imr = np.zeros((1080, 1920, 4), np.float32)

@@ -132,6 +132,26 @@ typedef struct vr_analytic {
  * (Scene::add_analytic_surface, scene.rs:118-124) */
 #define VR_SURFACE_ANALYTIC 0x80000000u
 
+/* An instance: mesh `mesh` placed by the affine map world = M p + c, with M
+ * the row-major 3x3 part of `transform` (elements 0-2, 4-6, 8-10) and c its
+ * last column (3, 7, 11).  surface_mesh[i] = VR_SURFACE_INSTANCE | k makes
+ * surface i instance k.  All instances of a mesh share one acceleration
+ * structure in object space; the ray enters it through the inverse map, so t
+ * keeps its world-space meaning; normals go through the inverse transpose.
+ * No counterpart in the reference (every surface there is its own mesh):
+ * scenes with instances render with the extended estimator only (the
+ * reference estimator refuses them).  Every engine traces them: inside an
+ * instance the wide engines take every candidate the conservative boxes
+ * reach (no reference reach test), the exact engine walks the mesh's
+ * median-split tree with the mapped ray.  Prim ids count an instance's
+ * triangles like a mesh surface's.  The transform must be finite and
+ * invertible (VR_ERR_INVALID otherwise). */
+#define VR_SURFACE_INSTANCE 0x40000000u
+typedef struct vr_instance {
+    uint32_t mesh;
+    float transform[12];
+} vr_instance;
+
 /* Whole scene (core/scene.rs:36-44).  Surface i is mesh surface_mesh[i]; as in
  * SceneAcceleration::hit (scene.rs:182-185) the material of a hit on surface
  * i is objects[i].material, so objects must have at least num_surfaces
@@ -152,6 +172,8 @@ typedef struct vr_scene_desc {
     uint32_t build_flags; /* VR_BUILD_* */
     uint32_t num_analytic;
     const vr_analytic* analytic;
+    uint32_t num_instances;
+    const vr_instance* instances;
 } vr_scene_desc;
 
 #define VR_BUILD_DEFAULT 0

@@ -294,6 +294,10 @@ class OracleScene:
     """The scene restated on the CPU (f32 = reference parity, f64 = convergence oracle)."""
 
     def __init__(self, scene, precision="f32"):
+        # the reference has no instancing: an instanced scene is restated as
+        # the world-space meshes it stands for (Scene.flattened)
+        if getattr(scene, "instances", None):
+            scene = scene.flattened()
         self.precision = precision
         desc, self._keep = scene.to_desc()
         f = lib().orc_scene_new_f32 if precision == "f32" else lib().orc_scene_new_f64

@@ -139,6 +139,16 @@ pub struct vr_analytic {
 }
 
 pub const VR_SURFACE_ANALYTIC: u32 = 0x8000_0000;
+pub const VR_SURFACE_INSTANCE: u32 = 0x4000_0000;
+
+/// An instance of a mesh under an affine map (row-major 3x4, object to world);
+/// no counterpart in the reference.
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct vr_instance {
+    pub mesh: u32,
+    pub transform: [f32; 12],
+}
 
 /// Whole scene (core/scene.rs:36-44); the material of surface i is objects[i].material.
 #[repr(C)]
@@ -159,6 +169,8 @@ pub struct vr_scene_desc {
     pub build_flags: u32,
     pub num_analytic: u32,
     pub analytic: *const vr_analytic,
+    pub num_instances: u32,
+    pub instances: *const vr_instance,
 }
 
 pub const VR_BUILD_DEFAULT: u32 = 0;

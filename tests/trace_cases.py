@@ -1,7 +1,7 @@
 """Ray-query scenes shared by the GPU and host traversal tests."""
 import numpy as np
 
-from voidray_amd.scene import Materials, Mesh, Scene
+from voidray_amd.scene import Materials, Mesh, Scene, Surfaces
 
 
 def edge_case_scene():
@@ -136,3 +136,60 @@ def bruteforce_lowest_handle(scene, o, d):
         base += len(ids)
     best_t = np.where(best_p == 0xFFFFFFFF, np.float32(0), best_t)
     return dict(prim=best_p, t=best_t, surface=best_s)
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_scene(n_instances=24, seed=11):
+    """Instances of two meshes (a UV sphere with vertex normals, a closed
+    six-face box) under random rotations, non-uniform scales (up to 3:1) and,
+    for every fifth one, a mirror (det < 0); beside them an ordinary mesh
+    surface and an analytic sphere.  Returns (scene, origins, directions) with
+    rays from random outside points aimed at random points of random placed
+    triangles (most rays hit an instance)."""
+    from voidray_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    scene = Scene.empty()
+    mat = scene.add_material(Materials.lambertian((0.5, 0.5, 0.5)))
+    sphere = scene.add_instanced_mesh(W.uv_sphere((0.0, 0.0, 0.0), 1.0, 24, 12))
+    c = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
+    faces = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 4, 5, 1], [2, 3, 7, 6], [0, 2, 6, 4], [1, 5, 7, 3]]
+    box_idx = np.array([[f[0], f[1], f[2], f[0], f[2], f[3]] for f in faces], np.uint32).reshape(-1, 3)
+    box = scene.add_instanced_mesh(Mesh(c, box_idx))
+    plain = scene.add_mesh(W.uv_sphere((0.0, -4.0, 0.0), 0.7, 16, 8))
+    scene.add_object(mat, plain)
+    scene.add_object(mat, scene.add_analytic_surface(Surfaces.sphere((4.0, 4.0, -4.0), 0.8)))
+    for k in range(n_instances):
+        R = _rotation(rng)
+        S = np.diag(rng.uniform(0.25, 0.75, 3))
+        if k % 5 == 4:
+            S[0, 0] = -S[0, 0]
+        M = R @ S
+        t = rng.uniform(-4, 4, 3)
+        xf = np.c_[M, t].astype(np.float32)
+        scene.add_object(mat, scene.add_instance(sphere if k % 2 == 0 else box, xf))
+    flat = scene.flattened()
+    # targets: random points of random triangles of the placed surfaces
+    tris = []
+    for sm in flat.surfaces:
+        if sm & 0x80000000:
+            continue
+        m = flat.meshes[sm]
+        tris.append(m.positions[m.indices].astype(np.float64))
+    tris = np.concatenate(tris)
+    n = 6000
+    pick = tris[rng.integers(0, len(tris), n)]
+    b = rng.dirichlet(np.ones(3), n)
+    tgt = np.einsum("nk,nkc->nc", b, pick)
+    src = rng.normal(size=(n, 3))
+    src = 12.0 * src / np.linalg.norm(src, axis=1, keepdims=True)
+    d = tgt - src
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return scene, src.astype(np.float32), W._normalize_rows(d.astype(np.float32))

@@ -18,6 +18,7 @@ VR_ENV_NONE, VR_ENV_UNIFORM, VR_ENV_HDRI = 0, 1, 2
 VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF, VR_BUILD_LEAF_RECORDS, VR_BUILD_LEAF_REFS = 0, 1, 2, 4
 VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
 VR_SURFACE_ANALYTIC = 0x80000000
+VR_SURFACE_INSTANCE = 0x40000000
 VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
 VR_ENGINE_WIDE, VR_ENGINE_REFERENCE, VR_ENGINE_WATERTIGHT = 0, 1, 2
 VR_NOALLOC_MAX_RAYS = 1048576
@@ -75,6 +76,10 @@ class vr_analytic(C.Structure):
     _fields_ = [("kind", C.c_int32), ("center", C.c_float * 3), ("radius", C.c_float), ("height", C.c_float)]
 
 
+class vr_instance(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("transform", C.c_float * 12)]
+
+
 class vr_scene_desc(C.Structure):
     _fields_ = [("num_meshes", C.c_uint32), ("meshes", C.POINTER(vr_mesh)),
                 ("num_surfaces", C.c_uint32), ("surface_mesh", C.c_void_p),
@@ -82,7 +87,8 @@ class vr_scene_desc(C.Structure):
                 ("num_materials", C.c_uint32), ("materials", C.POINTER(vr_material)),
                 ("num_textures", C.c_uint32), ("textures", C.POINTER(vr_texture)),
                 ("environment", vr_environment), ("camera", vr_camera), ("build_flags", C.c_uint32),
-                ("num_analytic", C.c_uint32), ("analytic", C.POINTER(vr_analytic))]
+                ("num_analytic", C.c_uint32), ("analytic", C.POINTER(vr_analytic)),
+                ("num_instances", C.c_uint32), ("instances", C.POINTER(vr_instance))]
 
 
 class vr_scene_stats(C.Structure):

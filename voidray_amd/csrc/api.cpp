@@ -203,13 +203,14 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
             const bool wt = desc->engine == VR_ENGINE_WATERTIGHT;
             bool exact = desc->engine == VR_ENGINE_REFERENCE;
             if (!exact) {
-                int r;
-                if (wt)
-                    r = any ? vr::trace_wide<kStack, true, true>(sv, oi, di, desc->t_min, kInf, ws, h)
-                            : vr::trace_wide<kStack, false, true>(sv, oi, di, desc->t_min, kInf, ws, h);
-                else
-                    r = any ? vr::trace_wide<kStack, true>(sv, oi, di, desc->t_min, kInf, ws, h)
-                            : vr::trace_wide<kStack, false>(sv, oi, di, desc->t_min, kInf, ws, h);
+                // the kernels' template variant: [any + 2 * watertight + 4 * instances]
+                int (*const wide[8])(const vr::SceneView&, vf3, vf3, float, float, const vr::WideStack&, vr::HitOut&) = {
+                    vr::trace_wide<kStack, false, false, false>, vr::trace_wide<kStack, true, false, false>,
+                    vr::trace_wide<kStack, false, true, false>,  vr::trace_wide<kStack, true, true, false>,
+                    vr::trace_wide<kStack, false, false, true>,  vr::trace_wide<kStack, true, false, true>,
+                    vr::trace_wide<kStack, false, true, true>,   vr::trace_wide<kStack, true, true, true>};
+                const int r = wide[(any ? 1 : 0) + (wt ? 2 : 0) + (sv.num_instances ? 4 : 0)](sv, oi, di, desc->t_min,
+                                                                                             kInf, ws, h);
                 if (r == vr::kStepFallback) {
                     exact = true;
                     ++fb;

@@ -220,9 +220,26 @@ static int clip_half(const DV3* in, int n, int axis, double plane, bool below, D
     return m;
 }
 
+// A top-level entry standing for an instance: p0/p1 hold its world box.
+static inline bool is_instance_rec(const TriRec& t) {
+    uint32_t m;
+    std::memcpy(&m, &t.p2[0], 4);
+    return m == kInstanceMarker;
+}
+
 // Float box (rounded outward, padded) of the part of triangle `t` inside `box`
-// (closed); false when that part is empty.  Never larger than `box`.
+// (closed); false when that part is empty.  Never larger than `box`.  An
+// instance entry's part is its box clipped to `box` (its geometry lies inside
+// its box).
 static bool clipped_box(const TriRec& t, const BBox& box, BBox& out) {
+    if (is_instance_rec(t)) {
+        for (int k = 0; k < 3; ++k) {
+            out.lo[k] = std::max(t.p0[k], box.lo[k]);
+            out.hi[k] = std::min(t.p1[k], box.hi[k]);
+            if (!(out.lo[k] <= out.hi[k])) return false;
+        }
+        return true;
+    }
     DV3 a[kMaxPoly], b[kMaxPoly];
     for (int k = 0; k < 3; ++k) {
         a[0].v[k] = t.p0[k];
@@ -553,7 +570,7 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
         b.empty();
         b.grow(tris_in[i].p0);
         b.grow(tris_in[i].p1);
-        b.grow(tris_in[i].p2);
+        if (!is_instance_rec(tris_in[i])) b.grow(tris_in[i].p2);
         pad_box(b);
         refs[i] = SRef{b, (uint32_t)i};
     }

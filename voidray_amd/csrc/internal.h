@@ -68,6 +68,23 @@ struct MeshInfo {
     uint32_t vert_base;    // first vertex of the mesh in the concatenated vertex arrays
 };
 
+// An instance (vr_instance): the placement of a mesh, its inverse, the
+// inverse transpose for normals, and the object-space wide BVH its rays walk.
+// In the top-level wide BVH an instance is one leaf entry: a TriRec whose p0
+// and p1 hold its world box and whose p2.x holds kInstanceMarker (a
+// signalling-NaN pattern no vertex keeps: the build quiets such vertices), p2.y
+// the instance index.
+struct alignas(16) InstanceDev {
+    float xf[12];          // object -> world, row-major 3x4
+    float inv[12];         // world -> object
+    float nrm[9];          // (M^-1)^T, row-major
+    uint32_t mesh, surface, prim_base, blas_root;  // blas_root: QNode index
+    float face_sign;       // sign of det M: the transformed winding's face normal is face_sign (M^-1)^T n
+    uint32_t pad[2];
+};
+static_assert(sizeof(InstanceDev) == 160, "InstanceDev layout");
+constexpr uint32_t kInstanceMarker = 0x7FA1B2C3u;
+
 struct SurfaceInfo {
     uint32_t mesh;         // mesh index (analytic surfaces: kNoMesh)
     uint32_t prim_base;    // global prim id of the surface's first triangle
@@ -75,6 +92,7 @@ struct SurfaceInfo {
     uint32_t path_off;     // top-level ancestors: indices into top_path
     uint32_t path_len;
     int32_t analytic;      // index into SceneView::analytic, -1 for a mesh surface
+    int32_t instance;      // index into SceneView::instances, -1 otherwise (mesh: the instanced mesh)
 };
 constexpr uint32_t kNoMesh = 0xFFFFFFFFu;
 
@@ -124,6 +142,8 @@ struct SceneView {
     const int32_t* prim_anc;      // global prim -> parent of its leaf in the mesh tree (-1: none)
     const AnalyticDev* analytic;  // analytic surfaces, in surface order
     uint32_t num_analytic;
+    const InstanceDev* instances;
+    uint32_t num_instances;
     const GpuMaterial* materials;
     const GpuTexture* textures;
     const uint8_t* tex_data;
@@ -170,7 +190,7 @@ struct vr_scene {
     uint32_t fallback_cap = 0;
     u32x2* spill = nullptr;             // per-lane spill stacks of the persistent traversal
     uint32_t spill_lanes = 0;
-    uint32_t persist_blocks[4] = {0, 0, 0, 0};  // resident blocks of the persistent kernels [any + 2 * watertight]
+    uint32_t persist_blocks[8] = {};  // resident blocks of the persistent kernels [any + 2 * watertight + 4 * instances]
     hipEvent_t trace_ev[2] = {nullptr, nullptr};  // around the wide kernel of the last vr_trace*
     bool trace_timed = false;
     uint32_t last_fallbacks = 0;
@@ -191,6 +211,7 @@ struct HostScene {
     std::vector<MeshInfo> meshes;
     std::vector<SurfaceInfo> surfaces;
     std::vector<AnalyticDev> analytic;
+    std::vector<InstanceDev> instances;
     std::vector<float> face_n, vpos, vnrm, vuv, env, env_marg, env_cond, env_pdf;
     std::vector<uint32_t> env_marg_guide, env_cond_guide;
     std::vector<GpuMaterial> materials;

@@ -620,6 +620,9 @@ int validate(const vr_scene* s, const vr_render_desc& d, uint32_t chunk_paths) {
     if (d.engine != VR_ENGINE_WIDE && d.engine != VR_ENGINE_REFERENCE && d.engine != VR_ENGINE_WATERTIGHT)
         return fail(VR_ERR_INVALID, "bad engine");
     bool ext = is_ext(d);
+    if (s->view.num_instances && !ext)
+        return fail(VR_ERR_INVALID, "scenes with instances render with the extended estimator only "
+                                    "(flags ENV_IS and/or RUSSIAN_ROULETTE): the reference has no instancing");
     if (d.engine == VR_ENGINE_WATERTIGHT && !ext)
         return fail(VR_ERR_INVALID, "the watertight engine renders with the extended estimator only "
                                     "(flags ENV_IS and/or RUSSIAN_ROULETTE): its hits are not the reference's");

@@ -48,6 +48,17 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
         return fail(VR_ERR_INVALID, "objects[] must cover every surface index (scene.rs:184 looks up objects[surface])");
     for (uint32_t i = 0; i < d->num_surfaces; ++i) {
         const uint32_t sm = d->surface_mesh[i];
+        if (!(sm & VR_SURFACE_ANALYTIC) && (sm & VR_SURFACE_INSTANCE)) {
+            const uint32_t k = sm & ~VR_SURFACE_INSTANCE;
+            if (d->num_instances >= (1u << 29)) return fail(VR_ERR_INVALID, "too many instances (2^29)");
+            if (!d->instances || k >= d->num_instances) return fail(VR_ERR_INVALID, "instance index out of range");
+            if (!d->meshes || d->instances[k].mesh >= d->num_meshes)
+                return fail(VR_ERR_INVALID, "instance mesh index out of range");
+            for (int e = 0; e < 12; ++e)
+                if (!std::isfinite(d->instances[k].transform[e]))
+                    return fail(VR_ERR_INVALID, "instance transform is not finite");
+            continue;
+        }
         if (sm & VR_SURFACE_ANALYTIC) {
             const uint32_t k = sm & ~VR_SURFACE_ANALYTIC;
             if (!d->analytic || k >= d->num_analytic) return fail(VR_ERR_INVALID, "analytic surface index out of range");
@@ -162,8 +173,31 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     uint32_t S = d->num_surfaces;
     std::vector<Box3> sb(S);
     auto is_analytic = [&](uint32_t i) { return (d->surface_mesh[i] & VR_SURFACE_ANALYTIC) != 0; };
-    auto surface_prims = [&](uint32_t i) { return is_analytic(i) ? 1u : minfo[d->surface_mesh[i]].num_tris; };
+    auto is_instance = [&](uint32_t i) { return !is_analytic(i) && (d->surface_mesh[i] & VR_SURFACE_INSTANCE) != 0; };
+    auto surface_mesh_of = [&](uint32_t i) {
+        return is_instance(i) ? d->instances[d->surface_mesh[i] & ~VR_SURFACE_INSTANCE].mesh : d->surface_mesh[i];
+    };
+    auto surface_prims = [&](uint32_t i) { return is_analytic(i) ? 1u : minfo[surface_mesh_of(i)].num_tris; };
+    // world-space vertices of an instance, with the flattening's f32 operation order
+    auto xform = [](const float* xf, vf3 p) {
+        return mk3(((xf[0] * p.x + xf[1] * p.y) + xf[2] * p.z) + xf[3],
+                   ((xf[4] * p.x + xf[5] * p.y) + xf[6] * p.z) + xf[7],
+                   ((xf[8] * p.x + xf[9] * p.y) + xf[10] * p.z) + xf[11]);
+    };
     for (uint32_t i = 0; i < S; ++i) {
+        if (is_instance(i)) {  // bounds of the transformed vertices, as the flattened mesh has them
+            const vr_instance& in = d->instances[d->surface_mesh[i] & ~VR_SURFACE_INSTANCE];
+            const MeshInfo& mi = minfo[in.mesh];
+            const uint32_t nv = d->meshes[in.mesh].num_vertices;
+            Box3 b{mk3(kInf, kInf, kInf), mk3(-kInf, -kInf, -kInf)};
+            for (uint32_t k = 0; k < nv; ++k) {
+                vf3 p = xform(in.transform, ld3(vpos.data(), mi.vert_base + k));
+                b.lo = mk3(fminf(b.lo.x, p.x), fminf(b.lo.y, p.y), fminf(b.lo.z, p.z));
+                b.hi = mk3(fmaxf(b.hi.x, p.x), fmaxf(b.hi.y, p.y), fmaxf(b.hi.z, p.z));
+            }
+            sb[i] = b;
+            continue;
+        }
         if (!is_analytic(i)) {
             sb[i] = mesh_bounds[d->surface_mesh[i]];
             continue;
@@ -191,11 +225,47 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     // the smallest prim id.
     uint32_t pb = 0;
     std::vector<AnalyticDev>& analytic = H.analytic;
+    std::vector<InstanceDev>& instances = H.instances;
     for (uint32_t i = 0; i < S; ++i) {
-        sinfo[i].mesh = is_analytic(i) ? kNoMesh : d->surface_mesh[i];
+        sinfo[i].mesh = is_analytic(i) ? kNoMesh : surface_mesh_of(i);
         sinfo[i].prim_base = pb;
         sinfo[i].material = d->objects[i].material;
         sinfo[i].analytic = -1;
+        sinfo[i].instance = -1;
+        if (is_instance(i)) {
+            const vr_instance& in = d->instances[d->surface_mesh[i] & ~VR_SURFACE_INSTANCE];
+            InstanceDev x{};
+            for (int e = 0; e < 12; ++e) x.xf[e] = in.transform[e];
+            // inverse of the 3x3 part and of the translation, in f64, rounded once
+            double m[3][3], iv[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) m[r][c] = in.transform[4 * r + c];
+            const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                               m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+            if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return fail(VR_ERR_INVALID, "singular instance transform");
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                    iv[r][c] = (m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1]) / det;
+                }
+            for (int r = 0; r < 3; ++r) {
+                double t = 0;
+                for (int c = 0; c < 3; ++c) {
+                    x.inv[4 * r + c] = (float)iv[r][c];
+                    t -= iv[r][c] * (double)in.transform[4 * c + 3];
+                    x.nrm[3 * r + c] = (float)iv[c][r];
+                }
+                x.inv[4 * r + 3] = (float)t;
+            }
+            x.face_sign = det < 0 ? -1.0f : 1.0f;
+            x.mesh = in.mesh;
+            x.surface = i;
+            x.prim_base = pb;
+            x.blas_root = kInvalidChild;
+            sinfo[i].instance = (int32_t)instances.size();
+            instances.push_back(x);
+        }
         if (is_analytic(i)) {
             const vr_analytic& an = d->analytic[d->surface_mesh[i] & ~VR_SURFACE_ANALYTIC];
             sinfo[i].analytic = (int32_t)analytic.size();
@@ -250,6 +320,29 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             prim_surface[sinfo[i].prim_base] = i;
             continue;
         }
+        if (sinfo[i].instance >= 0) {  // one top-level entry: the instance's world box
+            const MeshInfo& mi = minfo[sinfo[i].mesh];
+            for (uint32_t t = 0; t < mi.num_tris; ++t) prim_surface[sinfo[i].prim_base + t] = i;
+            if (mi.num_tris == 0) continue;
+            recs.emplace_back();
+            TriRec& r = recs.back();
+            const Box3& b = sb[i];
+            for (int a = 0; a < 3; ++a) {
+                // room for the rounding of the ray's map into object space
+                const float lo = comp(b.lo, a), hi = comp(b.hi, a);
+                const float pad = (std::fabs(lo) + std::fabs(hi) + (hi - lo)) * (1.0f / 1048576.0f) + 1e-30f;
+                r.p0[a] = lo - pad;
+                r.p1[a] = hi + pad;
+            }
+            const uint32_t k = (uint32_t)sinfo[i].instance, marker = kInstanceMarker;
+            std::memcpy(&r.p2[0], &marker, 4);
+            std::memcpy(&r.p2[1], &k, 4);
+            r.p2[2] = 0.f;
+            r.prim = sinfo[i].prim_base;
+            r.reserved = 0;
+            r.flags = i << 1;
+            continue;
+        }
         const MeshInfo& mi = minfo[sinfo[i].mesh];
         for (uint32_t t = 0; t < mi.num_tris; ++t) {
             recs.emplace_back();
@@ -263,6 +356,12 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             r.prim = sinfo[i].prim_base + t;
             r.reserved = 0;
             r.flags = (mi.root >= 0 ? 1u : 0u) | (i << 1);
+            uint32_t xb;
+            std::memcpy(&xb, &r.p2[0], 4);
+            if (xb == kInstanceMarker) {  // a vertex may not spell the instance marker: it is a NaN anyway
+                xb = 0x7FC00000u;
+                std::memcpy(&r.p2[0], &xb, 4);
+            }
             prim_surface[sinfo[i].prim_base + t] = i;
             prim_anc[sinfo[i].prim_base + t] = leaf_parent[mi.tri_base + t];
         }
@@ -272,12 +371,61 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     uint32_t depth = 0;
     std::string err;
     if (build_qbvh(recs, qn, tris, depth, err) != 0) return fail(VR_ERR_INVALID, err);
+    // one object-space wide BVH per instanced mesh, appended to the node and
+    // record arrays (its child links shifted accordingly)
+    if (!instances.empty()) {
+        std::vector<uint32_t> blas_of(M, kInvalidChild);
+        for (InstanceDev& x : instances) {
+            const MeshInfo& mi = minfo[x.mesh];
+            if (mi.num_tris == 0) continue;
+            if (blas_of[x.mesh] == kInvalidChild) {
+                std::vector<TriRec> brec(mi.num_tris);
+                for (uint32_t t = 0; t < mi.num_tris; ++t) {
+                    TriRec& r = brec[t];
+                    const uint32_t* ix = &midx[3 * (size_t)(mi.tri_base + t)];
+                    for (int a = 0; a < 3; ++a) {
+                        r.p0[a] = vpos[3 * (size_t)(mi.vert_base + ix[0]) + a];
+                        r.p1[a] = vpos[3 * (size_t)(mi.vert_base + ix[1]) + a];
+                        r.p2[a] = vpos[3 * (size_t)(mi.vert_base + ix[2]) + a];
+                    }
+                    uint32_t xb;
+                    std::memcpy(&xb, &r.p2[0], 4);
+                    if (xb == kInstanceMarker) {
+                        xb = 0x7FC00000u;
+                        std::memcpy(&r.p2[0], &xb, 4);
+                    }
+                    r.prim = t;  // within the mesh: the instance adds its prim base
+                    r.reserved = 0;
+                    r.flags = 0;
+                }
+                std::vector<QNode> bq;
+                std::vector<TriRec> bt;
+                uint32_t bdepth = 0;
+                if (build_qbvh(brec, bq, bt, bdepth, err) != 0) return fail(VR_ERR_INVALID, err);
+                const uint32_t qoff = (uint32_t)qn.size(), toff = (uint32_t)tris.size();
+                if ((uint64_t)toff + bt.size() >= (1ull << 27))
+                    return fail(VR_ERR_INVALID, "too many triangle references for the 27-bit leaf encoding");
+                for (QNode& q : bq)
+                    for (int c = 0; c < 4; ++c) {
+                        if (q.child[c] == kInvalidChild) continue;
+                        q.child[c] = (q.child[c] & kLeafBit) ? q.child[c] + toff : q.child[c] + qoff;
+                    }
+                qn.insert(qn.end(), bq.begin(), bq.end());
+                tris.insert(tris.end(), bt.begin(), bt.end());
+                blas_of[x.mesh] = qoff;
+                depth = std::max(depth, bdepth + 1);
+            }
+            x.blas_root = blas_of[x.mesh];
+        }
+    }
+    if (qn.size() >= (1u << 29)) return fail(VR_ERR_INVALID, "too many wide BVH nodes (2^29)");
     const size_t num_leaf_entries = tris.size();
     // Spatial splits copy a triangle into every leaf that holds a piece of it.
     // On request leaves hold 4-byte references into one record per triangle
     // (first-use order, so neighbouring leaves find their triangles close
     // together): less memory, one more dependent load per leaf entry.
-    const bool refs = (d->build_flags & VR_BUILD_LEAF_REFS) && !(d->build_flags & VR_BUILD_LEAF_RECORDS);
+    const bool refs = (d->build_flags & VR_BUILD_LEAF_REFS) && !(d->build_flags & VR_BUILD_LEAF_RECORDS) &&
+                      instances.empty();
     H.tri_ref.clear();
     if (refs) {
         std::vector<uint32_t> uniq_of_prim(T, 0xFFFFFFFFu);
@@ -380,6 +528,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     v.prim_anc = prim_anc.data();
     v.analytic = analytic.data();
     v.num_analytic = (uint32_t)analytic.size();
+    v.instances = instances.data();
+    v.num_instances = (uint32_t)instances.size();
     v.materials = mats.data();
     v.textures = texs.data();
     v.tex_data = texdata.data();
@@ -432,6 +582,7 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.ref_parent, &v.ref_parent);
     rc |= upload(s, H.prim_anc, &v.prim_anc);
     rc |= upload(s, H.analytic, &v.analytic);
+    rc |= upload(s, H.instances, &v.instances);
     rc |= upload(s, H.materials, &v.materials);
     rc |= upload(s, H.textures, &v.textures);
     rc |= upload(s, H.tex_data, &v.tex_data);

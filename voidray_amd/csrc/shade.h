@@ -318,6 +318,18 @@ VR_HD HitRec make_hit(const SceneView& sv, vf3 o, vf3 d, uint32_t prim, float t,
     h.uv.y = (u * sv.vuv[2 * v1 + 1] + v * sv.vuv[2 * v2 + 1]) + w * sv.vuv[2 * v0 + 1];
     const float* fp = sv.face_n + 3 * (size_t)(mi.tri_base + tri);
     vf3 fn = mk3(fp[0], fp[1], fp[2]);
+    if (si.instance >= 0) {
+        // an instance's mesh holds object-space normals: n' = (M^-1)^T n; the
+        // face normal of the placed triangle is that of its object-space
+        // normal, flipped when M reverses the winding
+        const InstanceDev& x = sv.instances[si.instance];
+        const float* q = x.nrm;
+        n = mk3((q[0] * n.x + q[1] * n.y) + q[2] * n.z, (q[3] * n.x + q[4] * n.y) + q[5] * n.z,
+                (q[6] * n.x + q[7] * n.y) + q[8] * n.z);
+        fn = mk3((q[0] * fn.x + q[1] * fn.y) + q[2] * fn.z, (q[3] * fn.x + q[4] * fn.y) + q[5] * fn.z,
+                 (q[6] * fn.x + q[7] * fn.y) + q[8] * fn.z);
+        fn = fn * (x.face_sign / mag(fn));
+    }
     const float thirty = (30.0f * kPi) / 180.0f;
     if (atan2f(mag(cross3(n, fn)), dot3(n, fn)) > thirty) n = fn;
     // HitRecord::new

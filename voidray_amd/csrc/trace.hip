@@ -85,7 +85,7 @@ static_assert(kFetchParts <= 64, "one lane clears each partition counter");
 // spatial-split BVH a ray needs only ~20 steps, so refills wait until a
 // good part of the wave is idle (24 of 64 lanes; 40 for the shorter any-hit
 // rays) and draw from partitioned counters (below).
-template <bool ANY, bool WT>
+template <bool ANY, bool WT, bool INST>
 __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
     __shared__ u32x2 lds[LDS * kWave];
@@ -158,15 +158,27 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                 }
                 continue;  // a second leaf waits in st.cur
             }
+            if (INST && (st.cur & kInstEntry)) {
+                // entering or leaving an instance changes the ray: a parked
+                // leaf is tested with the current one first (the entry waits)
+                if (leaf != kNone) continue;
+                st.have = false;
+                if (wide_instance<LDS, WT>(a.sv, st, ws, st.cur) == kStepFallback) fallback = true;
+                continue;
+            }
             st.have = false;
             if (wide_node<LDS, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
         }
         // Leaf phase
-        if (leaf != kNone && !fallback && wide_leaf<ANY, WT>(a.sv, st, leaf, a.tmin, a.tmax) == kStepDone) done = true;
+        if (leaf != kNone && !fallback) {
+            const int rc = wide_leaf<LDS, ANY, WT, INST>(a.sv, st, ws, leaf, a.tmin, a.tmax);
+            if (rc == kStepDone) done = true;
+            else if (rc == kStepFallback) fallback = true;
+        }
         if (live && !done && !fallback && !st.have && st.sp == 0) done = true;
         if (done || fallback) {
             HitOut h;
-            int rc = fallback ? kStepFallback : wide_finish<ANY, WT>(a.sv, st, a.tmin, h);
+            int rc = fallback ? kStepFallback : wide_finish<ANY, WT, INST>(a.sv, st, a.tmin, h);
             if (rc == kStepFallback) {
                 uint32_t slot = atomicAdd(a.fb_count, 1u);
                 a.fb_list[slot] = ray;
@@ -182,24 +194,41 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
 // stack then bounds residency at 6 waves; a 4 KB stack gave 8 waves and ~1%
 // more on short rays but spilled more on long ones).  Any hit keeps the
 // registers it wants (7 waves): held to 64 it spills.
-template <bool ANY, bool WT = false>
+template <bool ANY, bool WT = false, bool INST = false>
 __global__ void k_trace_persist(TraceArgs a);
 template <>
-__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, false>(TraceArgs a) {
-    trace_persist<false, false>(a);
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, false, false>(TraceArgs a) {
+    trace_persist<false, false, false>(a);
 }
 template <>
-__global__ __launch_bounds__(kWave) void k_trace_persist<true, false>(TraceArgs a) {
-    trace_persist<true, false>(a);
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, false, false>(TraceArgs a) {
+    trace_persist<true, false, false>(a);
 }
 // watertight engine (extended estimator only)
 template <>
-__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, true>(TraceArgs a) {
-    trace_persist<false, true>(a);
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, true, false>(TraceArgs a) {
+    trace_persist<false, true, false>(a);
 }
 template <>
-__global__ __launch_bounds__(kWave) void k_trace_persist<true, true>(TraceArgs a) {
-    trace_persist<true, true>(a);
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, true, false>(TraceArgs a) {
+    trace_persist<true, true, false>(a);
+}
+// scenes with instances
+template <>
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, false, true>(TraceArgs a) {
+    trace_persist<false, false, true>(a);
+}
+template <>
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, false, true>(TraceArgs a) {
+    trace_persist<true, false, true>(a);
+}
+template <>
+__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, true, true>(TraceArgs a) {
+    trace_persist<false, true, true>(a);
+}
+template <>
+__global__ __launch_bounds__(kWave) void k_trace_persist<true, true, true>(TraceArgs a) {
+    trace_persist<true, true, true>(a);
 }
 
 __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
@@ -310,6 +339,13 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                 } else {
                     SurfaceInfo si = sv.surfaces[surf];
                     MeshInfo mi = sv.meshes[si.mesh];
+                    // an instance's mesh is walked with the ray in its object space
+                    vf3 mo = o, md = d;
+                    WTRay mwt = wt;
+                    if (si.instance >= 0) {
+                        instance_ray(sv.instances[si.instance], o, d, mo, md);
+                        if (WT) mwt = wt_prepare(md);
+                    }
                     int32_t tri = -1;
                     if (kind == kCoopMesh) {
                         RefNode nd = sv.ref_nodes[idx];
@@ -317,7 +353,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                             tri = nd.obj;
                         } else {
                             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
-                            if (WT ? wt_slab_hit(b, o, d, a.tmin, kInf) : ref_slab_hit(b, o, d, a.tmin, kInf)) {
+                            if (WT ? wt_slab_hit(b, mo, md, a.tmin, kInf) : ref_slab_hit(b, mo, md, a.tmin, kInf)) {
                                 e0 = u32x2{(kCoopMesh << 30) | (uint32_t)nd.right, surf};
                                 e1 = u32x2{(kCoopMesh << 30) | (idx + 1), surf};
                                 np = 2;
@@ -333,7 +369,7 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
                         vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
                         MTResult r;
-                        if (WT ? wt_hit(wt, o, p0, p1, p2, a.tmin, r) : mt_hit(o, d, p0, p1, p2, a.tmin, r)) {
+                        if (WT ? wt_hit(mwt, mo, p0, p1, p2, a.tmin, r) : mt_hit(mo, md, p0, p1, p2, a.tmin, r)) {
                             const uint32_t prim = si.prim_base + (uint32_t)tri;
                             if (r.t < bt || (r.t == bt && prim < bprim)) {
                                 bt = r.t;
@@ -422,14 +458,20 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
     if (lane < kFetchParts) ctr[kCtrFetch + lane * kFetchStride] = 0u;
 }
 
+// The persistent kernel for a query: [any + 2 * watertight + 4 * instances].
+static void (*const kPersistKernels[8])(TraceArgs) = {
+    k_trace_persist<false, false, false>, k_trace_persist<true, false, false>, k_trace_persist<false, true, false>,
+    k_trace_persist<true, true, false>,   k_trace_persist<false, false, true>, k_trace_persist<true, false, true>,
+    k_trace_persist<false, true, true>,   k_trace_persist<true, true, true>};
+
 static int persist_grid(vr_scene* s, bool any, bool wt, uint32_t& blocks) {
-    const int k = (any ? 1 : 0) + (wt ? 2 : 0);
+    const bool inst = s->view.num_instances > 0;
+    const int k = (any ? 1 : 0) + (wt ? 2 : 0) + (inst ? 4 : 0);
     if (s->persist_blocks[k] == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         VR_HIP_CHECK(hipGetDevice(&dev));
         VR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        void (*const kern[4])(TraceArgs) = {k_trace_persist<false, false>, k_trace_persist<true, false>,
-                                             k_trace_persist<false, true>, k_trace_persist<true, true>};
+        void (*const* kern)(TraceArgs) = kPersistKernels;
         VR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern[k], kWave, 0));
         s->persist_blocks[k] = (uint32_t)std::max(1, cus * std::max(1, per_cu));
     }
@@ -491,8 +533,8 @@ static int launch_trace_with(vr_scene* s, const TraceJob& j, const TraceScratch&
         if (rc) return rc;
         blocks = std::min<uint32_t>(blocks, sc.spill_lanes / kWave);
         blocks = std::min<uint32_t>(blocks, (j.n_max + kWave - 1) / kWave);
-        void (*persist)(TraceArgs) = j.any ? (wt ? k_trace_persist<true, true> : k_trace_persist<true, false>)
-                                           : (wt ? k_trace_persist<false, true> : k_trace_persist<false, false>);
+        void (*persist)(TraceArgs) =
+            kPersistKernels[(j.any ? 1 : 0) + (wt ? 2 : 0) + (s->view.num_instances > 0 ? 4 : 0)];
         void (*exact)(TraceArgs, const uint32_t*, uint32_t*, uint32_t*, uint64_t*) =
             wt ? k_trace_exact_wave<true> : k_trace_exact_wave<false>;
         if (j.timed) VR_HIP_CHECK(hipEventRecord(s->trace_ev[0], stream));

@@ -30,6 +30,15 @@
 // triangle, the same rule).  Prim ids are numbered surface by surface in
 // triangle order, so both engines keep the lexicographic minimum of (t, prim).
 //
+// Instances (template flag INST, scenes with vr_instance surfaces; no
+// counterpart in the reference): an instance is one record of the top-level
+// wide BVH holding its world box (kInstanceMarker in p2.x).  A leaf that meets
+// one pushes an "enter" entry at the box's entry distance; popping it pushes an
+// "exit" entry, maps the ray into the instance's object space (t is unchanged:
+// the direction is mapped, not renormalised) and walks the shared object-space
+// BVH of its mesh; popping the exit entry restores the world ray.  Hits inside
+// an instance carry the instance's prim base and need no confirmation.
+//
 // Watertight mode (template flag WT, engine VR_ENGINE_WATERTIGHT): the same
 // traversal with the watertight ray/triangle test of vrmath.h in place of
 // Moller-Trumbore and no reference semantics: every candidate the
@@ -63,6 +72,17 @@ VR_HD void load_tri_geom(const TriRec* tris, uint32_t slot, vf3& p0, vf3& p1, vf
     p2 = mk3(b.z, b.w, c);
 }
 VR_HD uint32_t tri_word(const TriRec* tris, uint32_t slot, int w) { return ((const uint32_t*)(tris + slot))[w]; }
+
+// The ray (o, d) in an instance's object space: o' = M^-1 o + c', d' = M^-1 d
+// (not renormalised, so a hit's t is the world t), f32, the same operation
+// order in every engine.
+VR_HD void instance_ray(const InstanceDev& x, vf3 o, vf3 d, vf3& oo, vf3& dd) {
+    const float* m = x.inv;
+    oo = mk3(((m[0] * o.x + m[1] * o.y) + m[2] * o.z) + m[3], ((m[4] * o.x + m[5] * o.y) + m[6] * o.z) + m[7],
+             ((m[8] * o.x + m[9] * o.y) + m[10] * o.z) + m[11]);
+    dd = mk3((m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[4] * d.x + m[5] * d.y) + m[6] * d.z,
+             (m[8] * d.x + m[9] * d.y) + m[10] * d.z);
+}
 
 // Exact membership of a triangle in the reference's candidate set within its
 // mesh: BvhNode::hit (core/bvh.rs:132-181) descends into a child only after
@@ -189,7 +209,7 @@ constexpr uint32_t kAnalyticSlot = 0x40000000u;  // best_slot of an analytic win
 // (node or leaf code, bits of the entry distance) so a popped subtree that
 // lies beyond the current best hit is skipped without touching its node.
 struct WideState {
-    vf3 o, d;
+    vf3 o, d;  // the ray as the current BVH sees it (object space inside an instance)
     WTRay wt;  // watertight mode: the ray's shear (unused otherwise)
     float idx, idy, idz;
     float tbest, bu, bv;
@@ -197,7 +217,16 @@ struct WideState {
     uint32_t cur;
     int sp, lo;
     bool have, unconfirmed, found;
+    // INST only: the world ray, the instance being walked (kNoInstance: none)
+    // and its prim base, the instance of the best hit
+    vf3 wo, wd;
+    uint32_t inst, inst_base, best_inst;
 };
+constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
+// Stack codes of instance entries (INST): bit 30 set, below the leaf bit.
+// Node indices stay below 2^30 (the scene build checks it).
+constexpr uint32_t kInstEntry = 0x40000000u;            // | instance index: enter
+constexpr uint32_t kInstExit = kInstEntry | 0x3FFFFFFFu;  // leave the current instance
 
 struct WideStack {
     u32x2* lds;
@@ -209,7 +238,7 @@ struct WideStack {
 enum : int { kStepContinue = 0, kStepDone = 1, kStepFallback = 2 };
 
 template <bool WT = false>
-VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
+VR_HD void wide_set_ray(WideState& st, vf3 o, vf3 d) {
     const float kTiny = 1e-20f;
     st.o = o;
     st.d = d;
@@ -217,6 +246,16 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     st.idx = 1.0f / (fabsf(d.x) > kTiny ? d.x : copysignf(kTiny, d.x));
     st.idy = 1.0f / (fabsf(d.y) > kTiny ? d.y : copysignf(kTiny, d.y));
     st.idz = 1.0f / (fabsf(d.z) > kTiny ? d.z : copysignf(kTiny, d.z));
+}
+
+template <bool WT = false>
+VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
+    wide_set_ray<WT>(st, o, d);
+    st.wo = o;
+    st.wd = d;
+    st.inst = kNoInstance;
+    st.inst_base = 0;
+    st.best_inst = kNoInstance;
     st.tbest = kInf;
     st.bu = st.bv = 0.f;
     st.best_prim = 0xFFFFFFFFu;
@@ -293,11 +332,14 @@ VR_HD bool wide_pop_next(WideState& st, const WideStack& ws) {
     return false;
 }
 
-// Tests the triangles of leaf `code`.  Returns kStepContinue, or kStepDone
-// when an any-hit query has its confirmed hit.
-template <bool ANY, bool WT = false>
-VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmin, float tmax_any) {
+// Tests the triangles of leaf `code`.  Returns kStepContinue, kStepDone when
+// an any-hit query has its confirmed hit, or kStepFallback when an instance
+// entry finds the stack exhausted (INST).
+template <int LDS_STACK, bool ANY, bool WT = false, bool INST = false>
+VR_HD int wide_leaf(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code, float tmin,
+                    float tmax_any) {
     const vf3 o = st.o, d = st.d;
+    const bool in_inst = INST && st.inst != kNoInstance;
     uint32_t first = code & 0x07FFFFFFu;
     uint32_t cnt = ((code >> 27) & 7u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
@@ -305,17 +347,36 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
         const uint32_t slot = sv.tri_ref ? sv.tri_ref[first + k] : first + k;
         vf3 p0, p1, p2;
         load_tri_geom(sv.tris, slot, p0, p1, p2);
+        if (INST && f2u(p2.x) == kInstanceMarker) {
+            // an instance: its world box is [p0, p1]; enter it later, in
+            // near-to-far order with the other pending entries
+            const float tcull = ANY ? tmax_any : st.tbest * kCullSlack;
+            const float ax = (p0.x - o.x) * st.idx, bx = (p1.x - o.x) * st.idx;
+            const float ay = (p0.y - o.y) * st.idy, by = (p1.y - o.y) * st.idy;
+            const float az = (p0.z - o.z) * st.idz, bz = (p1.z - o.z) * st.idz;
+            const float tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+            const float tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tcull));
+            if (tnear <= tfar * kSlabSlack &&
+                !wide_push<LDS_STACK>(st, ws, u32x2{kInstEntry | f2u(p2.y), f2u(tnear)}))
+                return kStepFallback;
+            continue;
+        }
         MTResult r;
         if (WT ? !wt_hit(st.wt, o, p0, p1, p2, tmin, r) : !mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
         if (ANY) {
             if (r.t < tmax_any) {
-                if (WT || confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, slot, 11),
-                                         tri_word(sv.tris, slot, 9))) {
+                if (WT || in_inst ||
+                    confirm_winner(sv, o, d, tmin, r.t, p0, p1, p2, tri_word(sv.tris, slot, 11),
+                                   tri_word(sv.tris, slot, 9))) {
                     st.found = true;
                     st.tbest = r.t;
                     st.best_slot = slot;
                     st.bu = r.u;
                     st.bv = r.v;
+                    if (INST) {
+                        st.best_inst = st.inst;
+                        st.best_prim = st.inst_base + tri_word(sv.tris, slot, 9);
+                    }
                     return kStepDone;
                 }
                 st.unconfirmed = true;
@@ -323,15 +384,40 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, uint32_t code, float tmi
             continue;
         }
         if (r.t > st.tbest) continue;
-        const uint32_t prim = tri_word(sv.tris, slot, 9);
+        const uint32_t prim = (in_inst ? st.inst_base : 0u) + tri_word(sv.tris, slot, 9);
         if (r.t < st.tbest || prim < st.best_prim) {
             st.tbest = r.t;
             st.best_prim = prim;
             st.best_slot = slot;
             st.bu = r.u;
             st.bv = r.v;
+            if (INST) st.best_inst = st.inst;
         }
     }
+    return kStepContinue;
+}
+
+// Pops of instance entries (INST).  Enter: push the exit entry, map the world
+// ray into object space (instance_ray) and continue at the mesh BVH's root.  Exit: restore
+// the world ray.  Returns kStepContinue, or kStepFallback when the exit entry
+// does not fit on the stack.
+template <int LDS_STACK, bool WT = false>
+VR_HD int wide_instance(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code) {
+    if (code == kInstExit) {
+        wide_set_ray<WT>(st, st.wo, st.wd);
+        st.inst = kNoInstance;
+        return kStepContinue;
+    }
+    const uint32_t k = code & ~kInstEntry;
+    if (!wide_push<LDS_STACK>(st, ws, u32x2{kInstExit, 0u})) return kStepFallback;
+    const InstanceDev& x = sv.instances[k];
+    vf3 oo, dd;
+    instance_ray(x, st.wo, st.wd, oo, dd);
+    wide_set_ray<WT>(st, oo, dd);
+    st.inst = k;
+    st.inst_base = x.prim_base;
+    st.cur = x.blas_root;
+    st.have = true;
     return kStepContinue;
 }
 
@@ -403,21 +489,19 @@ VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uin
 
 // One traversal step (one node or one leaf).  Returns kStepContinue,
 // kStepDone (result ready for wide_finish) or kStepFallback.
-template <int LDS_STACK, bool ANY, bool WT = false>
+template <int LDS_STACK, bool ANY, bool WT = false, bool INST = false>
 VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
     if (!st.have && !wide_pop_next<LDS_STACK, ANY>(st, ws)) return kStepDone;
     st.have = false;
     const uint32_t cur = st.cur;
-    if (cur & kLeafBit) {
-        int rc = wide_leaf<ANY, WT>(sv, st, cur, tmin, tmax_any);
-        return rc == kStepDone ? kStepDone : kStepContinue;
-    }
+    if (cur & kLeafBit) return wide_leaf<LDS_STACK, ANY, WT, INST>(sv, st, ws, cur, tmin, tmax_any);
+    if (INST && (cur & kInstEntry)) return wide_instance<LDS_STACK, WT>(sv, st, ws, cur);
     return wide_node<LDS_STACK, ANY>(sv, st, ws, cur, tmin, tmax_any);
 }
 
 // Result of a finished query (after kStepDone).  Returns kStepDone or
 // kStepFallback (winner not confirmed / any-hit candidates unconfirmed).
-template <bool ANY, bool WT = false>
+template <bool ANY, bool WT = false, bool INST = false>
 VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitOut& out) {
     out.t = 0.f;
     out.u = 0.f;
@@ -429,22 +513,24 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
             out.u = st.bu;
             out.v = st.bv;
             out.prim = (st.best_slot & kAnalyticSlot) ? sv.analytic[st.best_slot & ~kAnalyticSlot].prim
-                                                      : tri_word(sv.tris, st.best_slot, 9);
+                       : (INST && st.best_inst != kNoInstance) ? st.best_prim
+                                                                : tri_word(sv.tris, st.best_slot, 9);
             return kStepDone;
         }
         return st.unconfirmed ? kStepFallback : kStepDone;
     }
     if (st.best_slot == 0xFFFFFFFFu) return kStepDone;
-    if (st.best_slot & kAnalyticSlot) {  // confirmed when it was seeded
+    if ((st.best_slot & kAnalyticSlot) || (INST && st.best_inst != kNoInstance)) {
+        // analytic: confirmed when it was seeded; instance: nothing to confirm
         out.t = st.tbest;
         out.u = st.bu;
         out.v = st.bv;
-        out.prim = sv.analytic[st.best_slot & ~kAnalyticSlot].prim;
+        out.prim = st.best_prim;
         return kStepDone;
     }
     vf3 p0, p1, p2;
     load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
-    if (!WT && !confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
+    if (!WT && !confirm_winner(sv, st.wo, st.wd, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
                                tri_word(sv.tris, st.best_slot, 9)))
         return kStepFallback;
     out.t = st.tbest;
@@ -455,20 +541,20 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
 }
 
 // Whole query for one ray (used by the host-side checks and simple kernels).
-template <int LDS_STACK, bool ANY, bool WT = false>
+template <int LDS_STACK, bool ANY, bool WT = false, bool INST = false>
 VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_any, const WideStack& ws,
                      HitOut& out) {
     WideState st;
     wide_init<WT>(st, o, d, sv.num_qnodes > 0);
     if (sv.num_analytic) wide_seed_analytic<ANY, WT>(sv, st, tmin, tmax_any);
     int rc;
-    while ((rc = wide_step<LDS_STACK, ANY, WT>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
+    while ((rc = wide_step<LDS_STACK, ANY, WT, INST>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
     }
     if (rc == kStepFallback) {
         out.prim = 0xFFFFFFFFu;
         return kStepFallback;
     }
-    return wide_finish<ANY, WT>(sv, st, tmin, out);
+    return wide_finish<ANY, WT, INST>(sv, st, tmin, out);
 }
 
 // Exact reference engine for one ray (closest hit).  The two traversal
@@ -514,6 +600,13 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
             continue;
         }
         MeshInfo mi = sv.meshes[si.mesh];
+        // an instance's mesh is walked with the ray in its object space
+        vf3 mo = o, md = d;
+        WTRay mwt = wt;
+        if (si.instance >= 0) {
+            instance_ray(sv.instances[si.instance], o, d, mo, md);
+            if (WT) mwt = wt_prepare(md);
+        }
         auto test = [&](uint32_t tri) {
             const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + tri);
             const float* vp = sv.vpos + 3 * (size_t)mi.vert_base;
@@ -521,7 +614,7 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
             vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
             vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
             MTResult r;
-            if (WT ? !wt_hit(wt, o, p0, p1, p2, tmin, r) : !mt_hit(o, d, p0, p1, p2, tmin, r)) return;
+            if (WT ? !wt_hit(mwt, mo, p0, p1, p2, tmin, r) : !mt_hit(mo, md, p0, p1, p2, tmin, r)) return;
             const uint32_t prim = si.prim_base + tri;
             if (!have || r.t < bt || (r.t == bt && prim < out.prim)) {
                 have = true;
@@ -546,7 +639,7 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
                 continue;
             }
             Box3 b{mk3(nd.lo[0], nd.lo[1], nd.lo[2]), mk3(nd.hi[0], nd.hi[1], nd.hi[2])};
-            if (!(WT ? wt_slab_hit(b, o, d, tmin, kInf) : ref_slab_hit(b, o, d, tmin, kInf))) continue;
+            if (!(WT ? wt_slab_hit(b, mo, md, tmin, kInf) : ref_slab_hit(b, mo, md, tmin, kInf))) continue;
             mstack[msp++ * stride] = nd.right;
             mstack[msp++ * stride] = mj + 1;
         }

@@ -241,7 +241,10 @@ class RadianceCache:
 
 
 def scene_bounds(scene):
-    """Axis-aligned bounds of every mesh vertex of a Scene (for the cache's position normalisation)."""
+    """Axis-aligned bounds of every mesh vertex of a Scene (for the cache's position normalisation);
+    instances count with their placed vertices."""
+    if getattr(scene, "instances", None):
+        scene = scene.flattened()
     lo = np.full(3, np.inf, np.float64)
     hi = np.full(3, -np.inf, np.float64)
     for m in scene.meshes:

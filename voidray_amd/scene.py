@@ -131,6 +131,33 @@ class Materials:
 
 # ---- geometry ----------------------------------------------------------------
 
+def instance_points(xf, p):
+    """world = M p + c for a 3x4 row-major transform, in f32 with the
+    library's operation order: ((m0 x + m1 y) + m2 z) + c."""
+    xf = np.asarray(xf, f32).reshape(3, 4)
+    p = np.asarray(p, f32)
+    out = np.empty_like(p)
+    for r in range(3):
+        out[:, r] = ((xf[r, 0] * p[:, 0] + xf[r, 1] * p[:, 1]) + xf[r, 2] * p[:, 2]) + xf[r, 3]
+    return out
+
+
+def instance_normal_matrix(xf):
+    """(M^-1)^T of the 3x3 part, computed in f64 and rounded to f32 once."""
+    m = np.asarray(xf, np.float64).reshape(3, 4)[:, :3]
+    return np.linalg.inv(m).T.astype(f32)
+
+
+def instance_normals(xf, n):
+    """n' = (M^-1)^T n in f32 with the library's operation order (not renormalised)."""
+    nm = instance_normal_matrix(xf)
+    n = np.asarray(n, f32)
+    out = np.empty_like(n)
+    for r in range(3):
+        out[:, r] = (nm[r, 0] * n[:, 0] + nm[r, 1] * n[:, 1]) + nm[r, 2] * n[:, 2]
+    return out
+
+
 class Mesh:
     """Triangle mesh (core/mesh.rs).  normals/uvs None = Vertex::position."""
 
@@ -331,6 +358,7 @@ class Scene:
         self.surfaces: List[int] = []  # surface -> mesh index, or VR_SURFACE_ANALYTIC | analytic index
         self.meshes: List[Mesh] = []
         self.analytic: List[AnalyticSurface] = []
+        self.instances: List[tuple] = []  # (mesh index, 3x4 row-major float32 transform)
         self.textures: List[Texture] = []
         self.environment: Optional[Environment] = None
 
@@ -353,6 +381,43 @@ class Scene:
         self.surfaces.append(N.VR_SURFACE_ANALYTIC | (len(self.analytic) - 1))
         return len(self.surfaces) - 1
 
+    def add_instanced_mesh(self, mesh):
+        """A mesh that only instances place in the scene (not a surface itself).
+        Returns its mesh index for add_instance.  No counterpart in the reference."""
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh_index, transform):
+        """Surface = mesh `mesh_index` under the affine map world = M p + c,
+        `transform` a 3x4 (row-major [M | c]).  All instances of a mesh share
+        one acceleration structure.  No counterpart in the reference: scenes
+        with instances render with the extended estimator only."""
+        xf = np.ascontiguousarray(transform, dtype=f32).reshape(3, 4)
+        self.instances.append((int(mesh_index), xf))
+        self.surfaces.append(N.VR_SURFACE_INSTANCE | (len(self.instances) - 1))
+        return len(self.surfaces) - 1
+
+    def flattened(self):
+        """The same scene with every instance replaced by its own world-space
+        mesh (what the reference, which has no instancing, would be given):
+        positions p' = M p + c, normals n' = (M^-1)^T n, in f32 with the
+        library's operation order; meshes keep their order, the instanced ones
+        are appended."""
+        import copy
+        out = copy.copy(self)
+        out.meshes = list(self.meshes)
+        out.surfaces = list(self.surfaces)
+        out.instances = []
+        for i, sm in enumerate(self.surfaces):
+            if not (sm & N.VR_SURFACE_INSTANCE) or (sm & N.VR_SURFACE_ANALYTIC):
+                continue
+            mi, xf = self.instances[sm & ~N.VR_SURFACE_INSTANCE]
+            m = self.meshes[mi]
+            out.meshes.append(Mesh(instance_points(xf, m.positions), m.indices,
+                                   None if m.normals is None else instance_normals(xf, m.normals), m.uvs))
+            out.surfaces[i] = len(out.meshes) - 1
+        return out
+
     def add_object(self, material, surface):
         self.objects.append((int(surface), int(material)))
         return len(self.objects) - 1
@@ -372,7 +437,14 @@ class Scene:
 
     @property
     def num_triangles(self):
-        return sum(self.meshes[m].num_triangles for m in self.surfaces if not m & N.VR_SURFACE_ANALYTIC)
+        n = 0
+        for m in self.surfaces:
+            if m & N.VR_SURFACE_ANALYTIC:
+                continue
+            if m & N.VR_SURFACE_INSTANCE:
+                m = self.instances[m & ~N.VR_SURFACE_INSTANCE][0]
+            n += self.meshes[m].num_triangles
+        return n
 
     def to_desc(self, build_flags=0):
         """Flat C description; returns (desc, keepalive)."""
@@ -422,10 +494,13 @@ class Scene:
         ana = (N.vr_analytic * max(1, len(self.analytic)))()
         for i, a in enumerate(self.analytic):
             ana[i] = N.vr_analytic(a.kind, N.F3(*a.center), a.radius, a.height)
+        inst = (N.vr_instance * max(1, len(self.instances)))()
+        for i, (mi, xf) in enumerate(self.instances):
+            inst[i] = N.vr_instance(mi, (C.c_float * 12)(*np.asarray(xf, f32).reshape(12).tolist()))
         desc = N.vr_scene_desc(len(self.meshes), meshes, len(self.surfaces), ptr(surf), len(self.objects), objs,
                                len(self.materials), mats, len(self.textures), texs, env, cam, build_flags,
-                               len(self.analytic), ana)
-        keep += [meshes, objs, mats, texs, ana]
+                               len(self.analytic), ana, len(self.instances), inst)
+        keep += [meshes, objs, mats, texs, ana, inst]
         return desc, keep
 
     def build_acceleration(self, ctx=None, build_flags=0):

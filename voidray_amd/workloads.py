@@ -109,11 +109,15 @@ def heightfield_mesh(h, uv_repeat=4.0, half=1.0):
 
 
 def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048, env_size=(2048, 1024),
-                    sun_peak=1e4, seed=4, scale=10.0):
+                    sun_peak=1e4, seed=4, scale=10.0, instanced=False):
     """fBm-displaced heightfield (2*grid^2 triangles), n_spheres UV spheres with
     materials 1/3 Lambertian, 1/3 metal (fuzz U(0,1)), 1/3 dielectric 1.5; the
     terrain is Lambertian with procedural RGBA8 albedo and normal textures; an
     RGB f32 sky with a sun lobe of peak `sun_peak`.
+
+    instanced=True: the spheres are instances of one unit UV sphere (scale r,
+    translation to the centre), the same random draws; such a scene renders
+    with the extended estimator only (the reference has no instancing).
 
     The scene spans [-scale, scale]^2.  At unit scale the 1024-grid triangles
     (edges ~2e-3) would all be invisible: Triangle::hit uses t_min = 1e-5 as
@@ -126,6 +130,7 @@ def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048
     s = scene.add_mesh(heightfield_mesh(h, half=scale))
     scene.add_object(scene.add_material(Materials.lambertian_texture(alb, nrm)), s)
     n = h.shape[0] - 1
+    unit = scene.add_instanced_mesh(uv_sphere((0.0, 0.0, 0.0), 1.0, *sphere_res)) if instanced else None
     for k in range(n_spheres):
         x, z = rng.uniform(-0.85, 0.85, 2)
         r = rng.uniform(0.02, 0.06)
@@ -139,7 +144,10 @@ def terrain_spheres(grid=1024, n_spheres=200, sphere_res=(64, 32), tex_size=2048
             mat = Materials.metal(rng.uniform(0.5, 0.95, 3), rng.uniform(0.0, 1.0))
         else:
             mat = Materials.dielectric(1.5)
-        s = scene.add_mesh(uv_sphere((x, y, z), r, *sphere_res))
+        if instanced:
+            s = scene.add_instance(unit, [[r, 0, 0, x], [0, r, 0, y], [0, 0, r, z]])
+        else:
+            s = scene.add_mesh(uv_sphere((x, y, z), r, *sphere_res))
         scene.add_object(scene.add_material(mat), s)
     scene.environment = Environments.hdri(procedural_sky(*env_size, seed=seed + 2, sun_peak=sun_peak))
     scene.camera = Camera.look_at((0.0, 0.9 * scale, 2.3 * scale), (0.0, 0.0, 0.1 * scale), (0.0, 1.0, 0.0),
