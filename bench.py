@@ -107,7 +107,8 @@ def _terrain(ctx):
 
 
 def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=8,
-               warm_size=(240, 135, 2), query_roughness=0.0):
+               warm_size=(240, 135, 2), query_roughness=0.3, max_resolution=1024, lr_grid=1e-2, lr_mlp=1e-3,
+               depths=(2, 1)):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
     + MIS) at width x height, judged against the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
@@ -195,9 +196,10 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
         return x[:hh, :ww, :3].reshape(hh // 8, 8, ww // 8, 8, 3).mean((1, 3))
 
     ref8 = box8(ref)
-    for depth in (2, 1):
+    for depth in depths:
         for trained in (True, False):
-            cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=2048)
+            cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=max_resolution,
+                                  lr_grid=lr_grid, lr_mlp=lr_mlp)
             for sw, sh, sspp in ((w, h, spf), (ww, wh, wspp)):  # untimed buffer sizing, no training
                 cache.render_frame(accel, scratch.data_ptr(), make_render_desc(sw, sh, sspp, sspp, st, seed=3, flags=ext,
                                                                                rr_depth=4),
@@ -249,6 +251,8 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
                           "mean_over_reference": round(float(plain_img.mean() / ref.mean()), 4),
                           "relmse_8x8": round(relmse(box8(plain_img), ref8), 6)}
     out["query_roughness"] = query_roughness
+    out["grid"] = {"log2_table": 19, "base_resolution": 16, "max_resolution": max_resolution, "lr_grid": lr_grid,
+                   "lr_mlp": lr_mlp, "warm_frames": warm_frames}
     out["roofline"] = _cache_roofline(ctx, torch, stream, lo, hi)
     return out
 
@@ -587,11 +591,19 @@ def main():
     ap.add_argument("--only-serving", action="store_true", help="run the batched-serving leg alone (prints it)")
     ap.add_argument("--no-cache", action="store_true", help="skip the learned-radiance-cache leg")
     ap.add_argument("--only-cache", action="store_true", help="run the learned-radiance-cache leg alone (prints it)")
-    ap.add_argument("--cache-query-roughness", type=float, default=0.0,
+    ap.add_argument("--cache-query-roughness", type=float, default=0.3,
                     help="cached paths end at the first vertex from the query depth on whose roughness input is "
                          "at least this (0: at the query depth whatever the material)")
     ap.add_argument("--cache-train-steps", type=int, default=8,
                     help="Adam steps per cached frame (disjoint slices of the frame's 2^16 samples)")
+    ap.add_argument("--cache-max-res", type=int, default=1024, help="finest hash-grid resolution")
+    ap.add_argument("--cache-lr", type=float, nargs=2, default=(1e-2, 1e-3), metavar=("GRID", "MLP"),
+                    help="Adam learning rates of the hash grid and the MLP")
+    ap.add_argument("--cache-depths", default="2,1", help="query depths to run (comma-separated)")
+    ap.add_argument("--cache-warm", type=int, default=8, help="training-only warm-up frames")
+    ap.add_argument("--nrc-probe", action="store_true",
+                    help="the learned cache's kernels alone (for profiler runs): inference over 2^20 queries and "
+                         "training steps over 2^16 samples, 11 launches each, timed with HIP events; prints one line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 closest-hit queries of the workload, then a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
@@ -613,8 +625,17 @@ def main():
         st = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(st)
         print(json.dumps({"radiance_cache": _cache_leg(ctx, torch, st, train_steps=args.cache_train_steps,
-                                                       query_roughness=args.cache_query_roughness)}),
+                                                       query_roughness=args.cache_query_roughness,
+                                                       max_resolution=args.cache_max_res, lr_grid=args.cache_lr[0],
+                                                       lr_mlp=args.cache_lr[1], warm_frames=args.cache_warm,
+                                                       depths=tuple(int(x) for x in args.cache_depths.split(",")))}),
               flush=True)
+        return
+    if args.nrc_probe:
+        st = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(st)
+        lo, hi = (-10.0, -3.0, -10.0), (10.0, 6.0, 10.0)  # the terrain scene's box
+        print(json.dumps({"radiance_cache_kernels": _cache_roofline(ctx, torch, st, lo, hi)}), flush=True)
         return
     if args.only_serving:
         st = torch.cuda.Stream(device=dev)
@@ -696,7 +717,10 @@ def main():
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
     serving = None if args.no_serving else _serving_leg(ctx, torch, stream, cpu=not args.no_cpu)
     cache = None if args.no_cache else _cache_leg(ctx, torch, stream, train_steps=args.cache_train_steps,
-                                                  query_roughness=args.cache_query_roughness)
+                                                  query_roughness=args.cache_query_roughness,
+                                                  max_resolution=args.cache_max_res, lr_grid=args.cache_lr[0],
+                                                  lr_mlp=args.cache_lr[1], warm_frames=args.cache_warm,
+                                                  depths=tuple(int(x) for x in args.cache_depths.split(",")))
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),
