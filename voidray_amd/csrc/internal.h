@@ -36,7 +36,7 @@ struct alignas(16) QNode {
 static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kInvalidChild = 0xFFFFFFFFu;
-constexpr uint32_t kMaxLeaf = 4;  // triangles per leaf (count-1 fits 3 bits)
+constexpr uint32_t kMaxLeaf = 3;  // triangles per leaf (count-1 fits 3 bits; 4 measured 0.7% slower)
 
 // Triangle record in leaf order (48 B): world-space vertices exactly as the
 // reference stores them and the global prim id.  Prim ids number the

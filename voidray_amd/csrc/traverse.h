@@ -216,6 +216,7 @@ struct WideState {
     uint32_t best_prim, best_slot;
     uint32_t cur;
     int sp, lo;
+    int top;  // LDS ring slot of entry sp (sp modulo the ring size, kept incrementally)
     bool have, unconfirmed, found;
     // INST only: the world ray, the instance being walked (kNoInstance: none)
     // and its prim base, the instance of the best hit
@@ -263,6 +264,7 @@ VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     st.cur = 0;
     st.sp = 0;
     st.lo = 0;
+    st.top = 0;
     st.have = nonempty;
     st.unconfirmed = false;
     st.found = false;
@@ -290,32 +292,32 @@ VR_HD void wide_seed_analytic(const SceneView& sv, WideState& st, float tmin, fl
     st.bv = v;
 }
 
-// Slot of stack entry i in the LDS ring (i >= 0).
-template <int LDS_STACK>
-VR_HD int ring_slot(int i) {
-    return (LDS_STACK & (LDS_STACK - 1)) == 0 ? (i & (LDS_STACK - 1)) : (int)((unsigned)i % (unsigned)LDS_STACK);
-}
-
+// The LDS part of the stack is a ring of LDS_STACK entries: entry i lives in
+// slot i mod LDS_STACK, and st.top tracks the slot of entry st.sp without a
+// division.  When the ring is full (sp - lo == LDS_STACK) the slot of entry sp
+// is that of the oldest LDS entry lo, which moves to the spill region first.
 template <int LDS_STACK>
 VR_HD bool wide_push(WideState& st, const WideStack& ws, u32x2 e) {
     if (st.sp - st.lo == LDS_STACK) {
         if (st.lo >= ws.spill_cap) return false;
-        ws.spill[st.lo * ws.spill_stride] = ws.lds[ring_slot<LDS_STACK>(st.lo) * ws.stride];
+        ws.spill[st.lo * ws.spill_stride] = ws.lds[st.top * ws.stride];
         ++st.lo;
     }
-    ws.lds[ring_slot<LDS_STACK>(st.sp) * ws.stride] = e;
+    ws.lds[st.top * ws.stride] = e;
     ++st.sp;
+    st.top = st.top + 1 == LDS_STACK ? 0 : st.top + 1;
     return true;
 }
 
 template <int LDS_STACK>
 VR_HD u32x2 wide_pop(WideState& st, const WideStack& ws) {
     --st.sp;
+    st.top = st.top == 0 ? LDS_STACK - 1 : st.top - 1;
     if (st.sp < st.lo) {
         st.lo = st.sp;
         return ws.spill[st.sp * ws.spill_stride];
     }
-    return ws.lds[ring_slot<LDS_STACK>(st.sp) * ws.stride];
+    return ws.lds[st.top * ws.stride];
 }
 
 // Pops the nearest pending entry that survives culling into st.cur; false
