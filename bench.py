@@ -420,13 +420,14 @@ def _render_legs(ctx, cpu=True, c3_spp=64):
     a3i.render(imi, make_render_desc(1920, 1080, 1, c3_spp, s3, seed=99, flags=ext, rr_depth=4))  # warm-up
     imi[:] = 0
     st = a3i.render(imi, make_render_desc(1920, 1080, c3_spp, c3_spp, s3, seed=1, flags=ext, rr_depth=4))
-    x, r = imi[..., :3].astype(np.float64), img3[..., :3].astype(np.float64)
     out["terrain_1080p_64spp_instanced"] = {
         "ms": round(st["kernel_ms"], 2), "samples_per_s": round(st["samples"] / (st["kernel_ms"] * 1e-3), 1),
         "mrays_per_s": round(st["segments"] / (st["kernel_ms"] * 1e3), 2), "segments": st["segments"],
         "exact_engine_rays": st["fallback_rays"], "instances": len(c3i.instances), "scene_build_s": round(build3i, 2),
         "device_bytes": a3i.stats()["device_bytes"], "device_bytes_flattened": a3.stats()["device_bytes"],
-        "relmse_vs_flattened_same_streams": float(np.mean((x - r) ** 2 / (r * r + 0.01))),
+        # the flattened spheres are generated in f64 and rounded, the instances
+        # place one f32 unit sphere: the same picture, not the same bits
+        "mean_over_flattened": round(float(imi[..., :3].mean() / img3[..., :3].mean()), 5),
         "roofline": _render_roofline(st, env_cdf=True)}
     a3i.release()
     # the same frame under the reference estimator (no RR, no env IS, clamp 3)

@@ -490,6 +490,10 @@ struct SpatialBuilder {
                     left.push_back(r);
                 } else if (r.box.lo[a] >= sp_plane) {
                     right.push_back(r);
+                } else if (is_instance_rec(tris[r.tri])) {
+                    // an instance is not cut: a copy in two leaves would walk
+                    // its whole object-space BVH twice
+                    (0.5f * (r.box.lo[a] + r.box.hi[a]) < sp_plane ? left : right).push_back(r);
                 } else {
                     BBox lb = r.box, rb = r.box, lp, rp;
                     lb.hi[a] = sp_plane;

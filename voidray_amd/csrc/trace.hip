@@ -163,7 +163,12 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                 // leaf is tested with the current one first (the entry waits)
                 if (leaf != kNone) continue;
                 st.have = false;
-                if (wide_instance<LDS, WT>(a.sv, st, ws, st.cur) == kStepFallback) fallback = true;
+                if (st.cur == kInstExit) {  // back to the world ray, reloaded
+                    wide_instance_exit<WT>(st, mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]),
+                                           mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]));
+                } else if (wide_instance_enter<LDS, WT>(a.sv, st, ws, st.cur) == kStepFallback) {
+                    fallback = true;
+                }
                 continue;
             }
             st.have = false;
@@ -213,9 +218,10 @@ template <>
 __global__ __launch_bounds__(kWave) void k_trace_persist<true, true, false>(TraceArgs a) {
     trace_persist<true, true, false>(a);
 }
-// scenes with instances
+// scenes with instances (the instance bookkeeping needs a few more
+// registers: held to 8 waves per SIMD the closest-hit kernels spill)
 template <>
-__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, false, true>(TraceArgs a) {
+__global__ __launch_bounds__(kWave, 7) void k_trace_persist<false, false, true>(TraceArgs a) {
     trace_persist<false, false, true>(a);
 }
 template <>
@@ -223,7 +229,7 @@ __global__ __launch_bounds__(kWave) void k_trace_persist<true, false, true>(Trac
     trace_persist<true, false, true>(a);
 }
 template <>
-__global__ __launch_bounds__(kWave, 8) void k_trace_persist<false, true, true>(TraceArgs a) {
+__global__ __launch_bounds__(kWave, 6) void k_trace_persist<false, true, true>(TraceArgs a) {
     trace_persist<false, true, true>(a);
 }
 template <>

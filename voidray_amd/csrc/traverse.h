@@ -218,9 +218,8 @@ struct WideState {
     int sp, lo;
     int top;  // LDS ring slot of entry sp (sp modulo the ring size, kept incrementally)
     bool have, unconfirmed, found;
-    // INST only: the world ray, the instance being walked (kNoInstance: none)
-    // and its prim base, the instance of the best hit
-    vf3 wo, wd;
+    // INST only: the instance being walked (kNoInstance: none) and its prim
+    // base, the instance of the best hit (the world ray is the caller's)
     uint32_t inst, inst_base, best_inst;
 };
 constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
@@ -252,8 +251,6 @@ VR_HD void wide_set_ray(WideState& st, vf3 o, vf3 d) {
 template <bool WT = false>
 VR_HD void wide_init(WideState& st, vf3 o, vf3 d, bool nonempty) {
     wide_set_ray<WT>(st, o, d);
-    st.wo = o;
-    st.wd = d;
     st.inst = kNoInstance;
     st.inst_base = 0;
     st.best_inst = kNoInstance;
@@ -399,28 +396,30 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, const WideStack& ws, uin
     return kStepContinue;
 }
 
-// Pops of instance entries (INST).  Enter: push the exit entry, map the world
-// ray into object space (instance_ray) and continue at the mesh BVH's root.  Exit: restore
-// the world ray.  Returns kStepContinue, or kStepFallback when the exit entry
-// does not fit on the stack.
+// Pops of instance entries (INST).  Enter (code kInstEntry | k, popped in
+// world space): push the exit entry, map the ray into object space
+// (instance_ray) and continue at the mesh BVH's root; returns kStepFallback
+// when the exit entry does not fit on the stack.  Exit (kInstExit): the caller
+// hands back the world ray (the state does not keep it, to save registers).
 template <int LDS_STACK, bool WT = false>
-VR_HD int wide_instance(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code) {
-    if (code == kInstExit) {
-        wide_set_ray<WT>(st, st.wo, st.wd);
-        st.inst = kNoInstance;
-        return kStepContinue;
-    }
+VR_HD int wide_instance_enter(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code) {
     const uint32_t k = code & ~kInstEntry;
     if (!wide_push<LDS_STACK>(st, ws, u32x2{kInstExit, 0u})) return kStepFallback;
     const InstanceDev& x = sv.instances[k];
     vf3 oo, dd;
-    instance_ray(x, st.wo, st.wd, oo, dd);
+    instance_ray(x, st.o, st.d, oo, dd);
     wide_set_ray<WT>(st, oo, dd);
     st.inst = k;
     st.inst_base = x.prim_base;
     st.cur = x.blas_root;
     st.have = true;
     return kStepContinue;
+}
+
+template <bool WT = false>
+VR_HD void wide_instance_exit(WideState& st, vf3 world_o, vf3 world_d) {
+    wide_set_ray<WT>(st, world_o, world_d);
+    st.inst = kNoInstance;
 }
 
 // Visits inner node `cur`: children in near-to-far order, the nearest becomes
@@ -491,13 +490,19 @@ VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uin
 
 // One traversal step (one node or one leaf).  Returns kStepContinue,
 // kStepDone (result ready for wide_finish) or kStepFallback.
+// (wo, wd): the query's world ray, for leaving an instance (INST).
 template <int LDS_STACK, bool ANY, bool WT = false, bool INST = false>
-VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any) {
+VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, float tmin, float tmax_any, vf3 wo,
+                    vf3 wd) {
     if (!st.have && !wide_pop_next<LDS_STACK, ANY>(st, ws)) return kStepDone;
     st.have = false;
     const uint32_t cur = st.cur;
     if (cur & kLeafBit) return wide_leaf<LDS_STACK, ANY, WT, INST>(sv, st, ws, cur, tmin, tmax_any);
-    if (INST && (cur & kInstEntry)) return wide_instance<LDS_STACK, WT>(sv, st, ws, cur);
+    if (INST && cur == kInstExit) {
+        wide_instance_exit<WT>(st, wo, wd);
+        return kStepContinue;
+    }
+    if (INST && (cur & kInstEntry)) return wide_instance_enter<LDS_STACK, WT>(sv, st, ws, cur);
     return wide_node<LDS_STACK, ANY>(sv, st, ws, cur, tmin, tmax_any);
 }
 
@@ -532,7 +537,8 @@ VR_HD int wide_finish(const SceneView& sv, const WideState& st, float tmin, HitO
     }
     vf3 p0, p1, p2;
     load_tri_geom(sv.tris, st.best_slot, p0, p1, p2);
-    if (!WT && !confirm_winner(sv, st.wo, st.wd, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
+    // (a finished query is back in world space: every exit entry was popped)
+    if (!WT && !confirm_winner(sv, st.o, st.d, tmin, st.tbest, p0, p1, p2, tri_word(sv.tris, st.best_slot, 11),
                                tri_word(sv.tris, st.best_slot, 9)))
         return kStepFallback;
     out.t = st.tbest;
@@ -550,7 +556,7 @@ VR_HD int trace_wide(const SceneView& sv, vf3 o, vf3 d, float tmin, float tmax_a
     wide_init<WT>(st, o, d, sv.num_qnodes > 0);
     if (sv.num_analytic) wide_seed_analytic<ANY, WT>(sv, st, tmin, tmax_any);
     int rc;
-    while ((rc = wide_step<LDS_STACK, ANY, WT, INST>(sv, st, ws, tmin, tmax_any)) == kStepContinue) {
+    while ((rc = wide_step<LDS_STACK, ANY, WT, INST>(sv, st, ws, tmin, tmax_any, o, d)) == kStepContinue) {
     }
     if (rc == kStepFallback) {
         out.prim = 0xFFFFFFFFu;
