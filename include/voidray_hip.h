@@ -255,7 +255,15 @@ int vr_trace(vr_scene* scene, const vr_trace_desc* desc,
              const float* origins, const float* directions,
              uint32_t* prim_id, float* t, float* u, float* v, uint8_t* hit);
 /* Same with device pointers, enqueued on `stream` (hipStream_t, NULL = the
- * context stream).  Any pointer among the outputs may be NULL. */
+ * context stream).  Any pointer among the outputs may be NULL.
+ * One stream per scene: every trace and render call on a scene uses that
+ * scene's device scratch (ray-fetch and fallback counters, fallback lists,
+ * traversal spill stacks), so calls on one scene that may overlap (other
+ * streams, threads) corrupt each other's results; enqueue them on one stream
+ * or synchronise between streams.  Different scenes are independent.  (The
+ * reference's SceneAcceleration::hit is &self: its callers may share one
+ * across threads; here each thread needs its own vr_scene or a common
+ * stream.) */
 int vr_trace_device_async(vr_scene* scene, const vr_trace_desc* desc,
                           const float* d_origins, const float* d_directions,
                           uint32_t* d_prim_id, float* d_t, float* d_u, float* d_v,
