@@ -740,7 +740,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": pmc.get("hbm_bytes_per_launch"),
-                     "kernel": "k_trace_persist<false>", "kernel_ms": round(kernel_ms, 4),
+                     "kernel": "k_trace_persist<false, false, false>", "kernel_ms": round(kernel_ms, 4),
                      "bytes_per_ray": BYTES_PER_RAY_CLOSEST,
                      # rocprofv3 PMC passes over the same workload (profiles/)
                      "traffic_bytes_per_ray": pmc.get("hbm_bytes_per_ray"),
