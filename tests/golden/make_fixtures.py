@@ -26,6 +26,10 @@ def c2_small():
 
 
 C2_FULL = dict(num_tris=1 << 20, seed=2, num_rays=1 << 22, res=2048, block=1 << 16)
+# the mid-size traversal fixture: 65,536 triangles, 262,144 rays
+C2_MID = dict(num_tris=1 << 16, seed=2, num_rays=1 << 18, res=512, block=1 << 14)
+# the tiny render fixture: cornell room 32x32, 4 spp, reference estimator
+C1_TINY = dict(width=32, height=32, spp=4, max_bounces=8, seed=21)
 
 
 def c2_full_digests(prim, t, u, v, block):
@@ -67,6 +71,42 @@ def c2_full():
     with open(os.path.join(HERE, "c2_full_digests.json"), "w") as f:
         json.dump(doc, f, indent=1)
         f.write("\n")
+
+
+def c2_mid_inputs():
+    cfg = C2_MID
+    scene = W.random_triangle_scene(cfg["num_tris"], seed=cfg["seed"])
+    o, d = W.microbench_rays(cfg["num_rays"], res=cfg["res"])
+    return scene, o, d
+
+
+def c2_mid():
+    """Oracle answers for the mid-size traversal fixture as per-block digests,
+    plus the number of hits (any hit = some closest hit, t_max = inf)."""
+    import json
+    cfg = C2_MID
+    scene, o, d = c2_mid_inputs()
+    r = oracle.OracleScene(scene).trace(o, d)
+    doc = dict(cfg, scene=scene_digest(scene), hits=int(np.sum(r["prim"] != 0xFFFFFFFF)),
+               blocks=c2_full_digests(r["prim"], r["t"], r["u"], r["v"], cfg["block"]))
+    with open(os.path.join(HERE, "c2_mid_digests.json"), "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+
+
+def c1_tiny_desc():
+    from voidray_amd.device import make_render_desc
+    from voidray_amd.scene import RenderSettings
+    cfg = C1_TINY
+    st = RenderSettings(total_samples=cfg["spp"], max_bounces=cfg["max_bounces"])
+    return make_render_desc(cfg["width"], cfg["height"], cfg["spp"], cfg["spp"], st, seed=cfg["seed"])
+
+
+def c1_tiny():
+    """The f32 restatement's image of the cornell room at 32x32, 4 spp."""
+    img, _, _ = oracle.OracleScene(W.cornell_room()).render(c1_tiny_desc())
+    np.savez_compressed(os.path.join(HERE, "c1_tiny.npz"), rgba=img.astype(np.float32),
+                        **{k: v for k, v in C1_TINY.items()})
 
 
 def scene_digest(scene):
@@ -141,7 +181,7 @@ def generator_digests():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["c2_small", "c2_full", "generator_digests"]
+    which = sys.argv[1:] or ["c2_small", "c2_mid", "c2_full", "c1_tiny", "generator_digests"]
     for name in which:
         globals()[name]()
     print("ok")

@@ -145,6 +145,17 @@ def test_emissive_only_is_clamped(gpu_ctx):
     assert np.allclose(img[..., :3], 3.0, rtol=0, atol=1e-6)
 
 
+def test_tiny_render_fixture(gpu_ctx):
+    """The committed image of the oracle's cornell room, 32x32, 4 spp
+    (tests/golden/c1_tiny.npz): the GPU gives it bit for bit."""
+    from tests.golden.make_fixtures import c1_tiny_desc
+    fx = np.load("tests/golden/c1_tiny.npz")
+    announce("render tiny fixture (cornell 32x32x4)")
+    img = np.zeros((32, 32, 4), np.float32)
+    W.cornell_room().build_acceleration(gpu_ctx).render(img, c1_tiny_desc())
+    assert np.array_equal(img, fx["rgba"]), _rel_err(img, fx["rgba"])
+
+
 def test_cornell_full_size_rows(gpu_ctx, oracle_lib):
     """The 512x512, 16 spp, depth 8 cornell configuration; every 16th row is
     checked against the oracle (the CPU restatement renders only those rows)."""

@@ -63,6 +63,26 @@ def test_golden_fixture(gpu_ctx):
     _compare(dict(prim=fx["prim"], t=fx["t"], u=fx["u"], v=fx["v"]), got, "golden")
 
 
+def test_mid_fixture(gpu_ctx):
+    """65,536 triangles, 262,144 rays (tests/golden/c2_mid_digests.json): the
+    wide and exact engines give the oracle's digests on every block; the
+    any-hit bits are the closest hits' and their count the fixture's."""
+    import json
+    from tests.golden.make_fixtures import c2_full_digests, c2_mid_inputs
+    with open("tests/golden/c2_mid_digests.json") as f:
+        fx = json.load(f)
+    scene, o, d = c2_mid_inputs()
+    acc = scene.build_acceleration(gpu_ctx)
+    for engine in (N.VR_ENGINE_WIDE, N.VR_ENGINE_REFERENCE):
+        announce(f"trace mid fixture engine {engine}")
+        r = acc.hit(o, d, engine=engine)
+        got = c2_full_digests(r["prim"], r["t"], r["u"], r["v"], fx["block"])
+        bad = [k for k, (a, b) in enumerate(zip(got, fx["blocks"])) if a != b]
+        assert not bad, f"engine {engine}: blocks {bad} differ"
+    anyh = acc.hit(o, d, any_hit=True)["hit"].astype(bool)
+    assert np.array_equal(anyh, r["prim"] != 0xFFFFFFFF) and int(anyh.sum()) == fx["hits"]
+
+
 def test_edge_cases(gpu_ctx, oracle_lib):
     scene, o, d = edge_case_scene()
     ref = oracle_lib.OracleScene(scene).trace(o, d)
