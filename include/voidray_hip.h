@@ -201,6 +201,9 @@ typedef struct vr_scene_stats {
     uint32_t num_tri_records;   /* triangle records in the wide BVH's leaves: >= num_triangles,
                                    a triangle cut by spatial splits has one per leaf holding a piece */
     uint32_t leaf_refs;         /* 1: leaves hold references into one record per triangle */
+    uint64_t qbox_checked;      /* build self-check: (child, axis) bounds of the wide nodes, decoded as the
+                                   kernels decode them, compared with the exact boxes they stand for */
+    uint64_t qbox_not_containing; /* ... of which did not contain the exact box (0 for a correct build) */
 } vr_scene_stats;
 
 /* ---- context ---------------------------------------------------------- */

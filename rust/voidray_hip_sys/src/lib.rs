@@ -193,6 +193,10 @@ pub struct vr_scene_stats {
     pub device_bytes: u64,
     pub num_tri_records: u32,
     pub leaf_refs: u32,
+    /// Build self-check: wide-node child bounds decoded as the kernels decode them ...
+    pub qbox_checked: u64,
+    /// ... of which did not contain the exact box they stand for (0 for a correct build).
+    pub qbox_not_containing: u64,
 }
 
 // ---- ray queries (SceneAcceleration::hit, core/scene.rs:182-185) ------------

@@ -96,7 +96,8 @@ class vr_scene_stats(C.Structure):
                 ("num_ref_nodes", C.c_uint32), ("qbvh_depth", C.c_uint32), ("max_leaf_size", C.c_uint32),
                 ("build_ref_seconds", C.c_double), ("build_sah_seconds", C.c_double),
                 ("upload_seconds", C.c_double), ("device_bytes", C.c_uint64),
-                ("num_tri_records", C.c_uint32), ("leaf_refs", C.c_uint32)]
+                ("num_tri_records", C.c_uint32), ("leaf_refs", C.c_uint32),
+                ("qbox_checked", C.c_uint64), ("qbox_not_containing", C.c_uint64)]
 
 
 class vr_trace_desc(C.Structure):

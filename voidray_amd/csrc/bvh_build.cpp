@@ -557,7 +557,7 @@ static int pick_exponent(float lo, float hi) {
 }  // namespace
 
 int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, std::vector<TriRec>& tris_out,
-               uint32_t& depth_out, std::string& err) {
+               uint32_t& depth_out, std::string& err, QbvhCheck* check) {
     size_t n = tris_in.size();
     qnodes.clear();
     tris_out.clear();
@@ -720,6 +720,20 @@ int build_qbvh(const std::vector<TriRec>& tris_in, std::vector<QNode>& qnodes, s
             }
         }
         qnodes[pw.qindex] = q;
+        if (check) {
+            // read back what was stored, decoded as the traversal decodes it
+            const QNode& w = qnodes[pw.qindex];
+            const double wo[3] = {w.ox, w.oy, w.oz};
+            const int we[3] = {w.ex, w.ey, w.ez};
+            for (int i = 0; i < nk; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    const double s = std::ldexp(1.0, we[a] - 127);
+                    const double lo = wo[a] + w.qlo[a][i] * s, hi = wo[a] + w.qhi[a][i] * s;
+                    const BBox& ex = bn[kids[i]].box;
+                    ++check->boxes_checked;
+                    if (!(lo <= (double)ex.lo[a] && hi >= (double)ex.hi[a])) ++check->boxes_not_containing;
+                }
+        }
     }
     return 0;
 }

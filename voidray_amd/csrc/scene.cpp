@@ -370,7 +370,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     std::vector<TriRec>& tris = H.tris;
     uint32_t depth = 0;
     std::string err;
-    if (build_qbvh(recs, qn, tris, depth, err) != 0) return fail(VR_ERR_INVALID, err);
+    QbvhCheck qcheck;
+    if (build_qbvh(recs, qn, tris, depth, err, &qcheck) != 0) return fail(VR_ERR_INVALID, err);
     // one object-space wide BVH per instanced mesh, appended to the node and
     // record arrays (its child links shifted accordingly)
     if (!instances.empty()) {
@@ -401,7 +402,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
                 std::vector<QNode> bq;
                 std::vector<TriRec> bt;
                 uint32_t bdepth = 0;
-                if (build_qbvh(brec, bq, bt, bdepth, err) != 0) return fail(VR_ERR_INVALID, err);
+                if (build_qbvh(brec, bq, bt, bdepth, err, &qcheck) != 0) return fail(VR_ERR_INVALID, err);
                 const uint32_t qoff = (uint32_t)qn.size(), toff = (uint32_t)tris.size();
                 if ((uint64_t)toff + bt.size() >= (1ull << 27))
                     return fail(VR_ERR_INVALID, "too many triangle references for the 27-bit leaf encoding");
@@ -548,6 +549,8 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
     st.qbvh_depth = depth;
     st.max_leaf_size = kMaxLeaf;
     st.num_tri_records = (uint32_t)num_leaf_entries;
+    st.qbox_checked = qcheck.boxes_checked;
+    st.qbox_not_containing = qcheck.boxes_not_containing;
     st.leaf_refs = H.tri_ref.empty() ? 0u : 1u;
     st.build_ref_seconds = std::chrono::duration<double>(t_ref1 - t_ref0).count();
     st.build_sah_seconds = std::chrono::duration<double>(t_sah1 - t_ref1).count();
