@@ -300,6 +300,11 @@ __device__ __forceinline__ vf3 fold_levels(const RenderArgs& a, uint32_t p, uint
 // (before the f64 libm helpers of the microfacet material stopped being
 // inlined): 233 VGPRs without spills at 2 waves 9% slower, 168 at 3 waves 3%
 // slower, 96 at 5 and 80 at 6 waves (more spills) 7% and 13% slower.
+//
+// EST: the estimators present in the launch (0: both, 1: reference only, 2:
+// extended only); CACHE: radiance-cache requests present.  A launch compiles
+// out what it does not run, which frees registers for what it does.
+template <int EST, bool CACHE>
 __global__ void k_shade(RenderArgs a, uint32_t depth) {
     const SceneView& sv = a.sv;
     const uint32_t n = depth == 0 ? a.npaths : a.cnt[kCntActive + depth];
@@ -320,6 +325,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
         if (act) {
             p = qin ? qin[k] : k;
             const ReqDev& r = a.reqs[a.path_req[p]];
+            const bool ext = EST == 2 || (EST == 0 && r.ext);
             const float c = r.clamp;
             const bool last = depth + 1 >= r.max_bounces;
             PathRng rng;
@@ -332,7 +338,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
             } else if (prim == 0xFFFFFFFFu) {
                 miss = true;
                 vf3 env = sv.env_kind == VR_ENV_NONE ? mk3(0.f, 0.f, 0.f) : env_lookup(sv, d);
-                if (!r.ext) {
+                if (!ext) {
                     st3(a.L, p, fold_levels(a, p, depth, env, c));
                 } else {
                     float w = 1.f;
@@ -348,7 +354,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                 HitRec hit = make_hit(sv, o, d, prim, a.hit_t[p], a.hit_u[p], a.hit_v[p]);
                 GpuMaterial m = sv.materials[sv.surfaces[hit.surface].material];
                 textured = m.albedo_tex >= 0 || m.normal_tex >= 0;
-                if (!r.ext) {
+                if (!ext) {
                     if (r.mode == VR_MODE_NORMAL) {
                         vf3 nn = normalize3(hit.normal);
                         vf3 att = mk3(0.5f * nn.x + 0.5f, 0.5f * nn.y + 0.5f, 0.5f * nn.z + 0.5f);
@@ -371,9 +377,9 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                 } else {
                     vf3 A = ld3(a.A, p), L = ld3(a.L, p);
                     const bool train_path =
-                        r.cmode == kCacheOn &&
+                        CACHE && r.cmode == kCacheOn &&
                         train_hash(a.path_pix[p], a.path_smp[p], r.request, r.frame) < r.train_threshold;
-                    const bool query = r.cmode == kCacheOn && !train_path && depth >= r.cache_depth &&
+                    const bool query = CACHE && r.cmode == kCacheOn && !train_path && depth >= r.cache_depth &&
                                        cache_roughness(m) >= r.query_roughness;
                     if (query || (train_path && depth < r.train_vertices)) {
                         // the vertex, its outgoing direction and the path state before its contributions
@@ -440,7 +446,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
             }
             a.rng_n[p] = rng.n;
         }
-        {
+        if (CACHE) {
             uint32_t slot_t = 0xFFFFFFFFu, slot_q = 0xFFFFFFFFu;
             if (a.tr.count)  // the same for the whole grid: a batch with radiance-cache requests
                 block_alloc2(rec == 1u, rec == 2u, a.tr.count, a.qr.count, slot_t, slot_q);
@@ -644,7 +650,7 @@ struct Chunk {
 struct Group {
     vr_scene* scene = nullptr;
     uint32_t engine = 0;
-    bool any_ext = false, any_env_is = false;
+    bool any_ext = false, any_ref = false, any_env_is = false;
     bool any_cached = false;  // radiance-cache requests (queries and training samples)
     uint32_t att_levels = 0;  // max bounces among reference-estimator requests
     uint32_t max_depth = 0;
@@ -776,6 +782,7 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
         Group& g = R.groups[f->second];
         g.reqs.push_back(i);
         g.any_ext |= r.ext != 0;
+        g.any_ref |= r.ext == 0;
         g.any_env_is |= r.ext && (d.flags & VR_RENDER_ENV_IS) && it.scene->view.env_kind == VR_ENV_HDRI;
         if (!r.ext) g.att_levels = std::max(g.att_levels, d.max_bounces);
         g.max_depth = std::max(g.max_depth, d.max_bounces);
@@ -990,7 +997,12 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                     j.fallback_accum = R.stats_dev + 3;
                     int rc = launch_trace(s, j, st);
                     if (rc) return rc;
-                    hipLaunchKernelGGL(k_shade, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a, depth);
+                    void (*shade)(RenderArgs, uint32_t) =
+                        g.any_cached ? (g.any_ref ? k_shade<0, true> : k_shade<2, true>)
+                                     : (g.any_ext && g.any_ref ? k_shade<0, false>
+                                        : g.any_ext           ? k_shade<2, false>
+                                                              : k_shade<1, false>);
+                    hipLaunchKernelGGL(shade, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a, depth);
                     if (g.any_env_is) {
                         TraceJob sj;
                         sj.o = a.sh_o;
