@@ -195,6 +195,7 @@ struct vr_scene {
     bool trace_timed = false;
     uint32_t last_fallbacks = 0;
     uint32_t num_prims = 0;
+    bool has_microfacet = false;  // a material of kind VR_MAT_MICROFACET (the shading kernel's variant)
 };
 
 namespace vr {

@@ -302,9 +302,10 @@ __device__ __forceinline__ vf3 fold_levels(const RenderArgs& a, uint32_t p, uint
 // slower, 96 at 5 and 80 at 6 waves (more spills) 7% and 13% slower.
 //
 // EST: the estimators present in the launch (0: both, 1: reference only, 2:
-// extended only); CACHE: radiance-cache requests present.  A launch compiles
-// out what it does not run, which frees registers for what it does.
-template <int EST, bool CACHE>
+// extended only); CACHE: radiance-cache requests present; MICRO: the scene
+// has MicrofacetBSDF materials.  A launch compiles out what it does not run,
+// which frees registers for what it does.
+template <int EST, bool CACHE, bool MICRO>
 __global__ void k_shade(RenderArgs a, uint32_t depth) {
     const SceneView& sv = a.sv;
     const uint32_t n = depth == 0 ? a.npaths : a.cnt[kCntActive + depth];
@@ -360,7 +361,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                         vf3 att = mk3(0.5f * nn.x + 0.5f, 0.5f * nn.y + 0.5f, 0.5f * nn.z + 0.5f);
                         st3(a.L, p, fold_levels(a, p, depth, mk3(0.f, 0.f, 0.f) + vminc(att, c), c));
                     } else {
-                        ScatterOut sc = scatter(sv, m, d, hit, rng, capped, false);
+                        ScatterOut sc = scatter<MICRO>(sv, m, d, hit, rng, capped, false);
                         if (!sc.has_ray) {
                             st3(a.L, p, fold_levels(a, p, depth, mk3(0.f, 0.f, 0.f) + vminc(sc.att, c), c));
                         } else {
@@ -419,7 +420,7 @@ __global__ void k_shade(RenderArgs a, uint32_t depth) {
                                 shadow = true;
                             }
                         }
-                        ScatterOut sc = scatter(sv, m, d, hit, rng, capped, true);
+                        ScatterOut sc = scatter<MICRO>(sv, m, d, hit, rng, capped, true);
                         if (sc.has_ray && !last) {
                             A = mulv(A, sc.att);
                             float pb = 0.f;
@@ -997,11 +998,16 @@ static int enqueue_batch(vr_ctx* ctx, hipStream_t st) {
                     j.fallback_accum = R.stats_dev + 3;
                     int rc = launch_trace(s, j, st);
                     if (rc) return rc;
-                    void (*shade)(RenderArgs, uint32_t) =
-                        g.any_cached ? (g.any_ref ? k_shade<0, true> : k_shade<2, true>)
-                                     : (g.any_ext && g.any_ref ? k_shade<0, false>
-                                        : g.any_ext           ? k_shade<2, false>
-                                                              : k_shade<1, false>);
+                    // [estimators][cache][microfacet materials]
+                    void (*const shades[3][2][2])(RenderArgs, uint32_t) = {
+                        {{k_shade<0, false, false>, k_shade<0, false, true>},
+                         {k_shade<0, true, false>, k_shade<0, true, true>}},
+                        {{k_shade<1, false, false>, k_shade<1, false, true>},
+                         {k_shade<0, true, false>, k_shade<0, true, true>}},  // (cache requests are extended)
+                        {{k_shade<2, false, false>, k_shade<2, false, true>},
+                         {k_shade<2, true, false>, k_shade<2, true, true>}}};
+                    const int est = g.any_ext && g.any_ref ? 0 : g.any_ext ? 2 : 1;
+                    void (*shade)(RenderArgs, uint32_t) = shades[est][g.any_cached][s->has_microfacet];
                     hipLaunchKernelGGL(shade, dim3(grid_for(c.npaths, sb)), dim3(sb), 0, st, a, depth);
                     if (g.any_env_is) {
                         TraceJob sj;

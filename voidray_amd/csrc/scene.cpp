@@ -597,6 +597,8 @@ int build_scene(vr_ctx* ctx, const vr_scene_desc* d, vr_scene* s) {
     rc |= upload(s, H.env_pdf, &v.env_pdf);
     if (rc) return VR_ERR_OOM;
     s->num_prims = v.num_tris;
+    s->has_microfacet = false;
+    for (const GpuMaterial& m : H.materials) s->has_microfacet |= m.kind == VR_MAT_MICROFACET;
     s->stats.upload_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

@@ -350,7 +350,10 @@ struct ScatterOut {
 
 // Material::scatter (voidray_common/src/simple.rs, core/traits.rs:19-41).
 // `cosine_lobe`: the extended estimator's Lambertian, which scatters about the
-// normalised shading normal so that its pdf is cos/pi exactly.
+// normalised shading normal so that its pdf is cos/pi exactly.  MICRO: the
+// scene has MicrofacetBSDF materials (compiled out otherwise: the caller
+// only passes false for scenes without them).
+template <bool MICRO = true>
 VR_HD ScatterOut scatter(const SceneView& sv, const GpuMaterial& m, vf3 d, const HitRec& hit, PathRng& rng,
                          uint32_t& capped, bool cosine_lobe) {
     ScatterOut s;
@@ -405,6 +408,11 @@ VR_HD ScatterOut scatter(const SceneView& sv, const GpuMaterial& m, vf3 d, const
             s.has_ray = false;
             return s;
         case VR_MAT_MICROFACET: {
+            if (!MICRO) {
+                s.att = mk3(0.f, 0.f, 0.f);
+                s.has_ray = false;
+                return s;
+            }
             // BSDFMaterial::scatter (traits.rs:19-41): wo is the ray's own
             // direction, normalised, as the reference passes it
             const vf3 wo = normalize3(d);
