@@ -106,7 +106,7 @@ def _terrain(ctx):
     return _TERRAIN["scene"], _TERRAIN["accel"]
 
 
-def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=8,
+def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=0,
                warm_size=(240, 135, 2), query_roughness=0.3, max_resolution=1024, lr_grid=1e-2, lr_mlp=1e-3,
                depths=(2, 1)):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
@@ -115,7 +115,8 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     Plain: progressive 4-spp passes, image snapshot after each.  Cached:
     `warm_frames` warm-up frames that only train (the same camera at
     warm_size = (w, h, spp), every path a training path, the image discarded;
-    counted in the cached run's time), then 32 frames of 4 spp.  In each frame
+    counted in the cached run's time; none by default: the cache learns from
+    the frames it renders), then 32 frames of 4 spp.  In each frame
     ~2^16 randomly chosen paths run to the end as training paths and record up
     to 4 vertices with their suffix radiance; the others stop at vertex
     `cache_depth` and add the cache's answer; 8 Adam steps on disjoint slices
@@ -601,7 +602,7 @@ def main():
     ap.add_argument("--cache-lr", type=float, nargs=2, default=(1e-2, 1e-3), metavar=("GRID", "MLP"),
                     help="Adam learning rates of the hash grid and the MLP")
     ap.add_argument("--cache-depths", default="2,1", help="query depths to run (comma-separated)")
-    ap.add_argument("--cache-warm", type=int, default=8, help="training-only warm-up frames")
+    ap.add_argument("--cache-warm", type=int, default=0, help="training-only warm-up frames")
     ap.add_argument("--nrc-probe", action="store_true",
                     help="the learned cache's kernels alone (for profiler runs): inference over 2^20 queries and "
                          "training steps over 2^16 samples, 11 launches each, timed with HIP events; prints one line")
