@@ -257,3 +257,31 @@ def test_trained_cache_gains_over_the_plain_tracer(gpu_ctx):
     assert abs(m["trained"][1] - 1.0) < 0.05 and m["untrained"][1] < 0.92
     assert m["trained"][2] < 0.35 * m["untrained"][2]
     assert m["trained"][2] < m["plain"][2]
+
+
+def test_cache_at_the_config_sizes_gains_at_equal_time(gpu_ctx):
+    """The radiance-cache leg at config 5's sizes (bench.py's own routine):
+    2^19-entry levels, 2^16 training paths per frame, the 2.9 M-triangle
+    terrain at 960x540 against the plain tracer at 4096 spp, queries at the
+    first rough vertex from bounce 2.  At the cached run's final time the
+    plain tracer's relMSE is higher (the config's metric), the trained image's
+    mean is within 2% of the reference while the untrained control is more
+    than 5% dark, and the trained cache's 8x8-box error is below half the
+    control's."""
+    import torch
+    import bench
+    announce("cache at config sizes: 960x540, 2^19 table, 2^16 training paths per frame (bench routine)")
+    st = torch.cuda.Stream(device="cuda")
+    torch.cuda.set_stream(st)
+    try:
+        out = bench._cache_leg(gpu_ctx, torch, st, depths=(2,))
+    finally:
+        torch.cuda.set_stream(torch.cuda.default_stream())
+    tr, un = out["cache_depth_2_trained"], out["cache_depth_2_untrained"]
+    print(f"[cache config] trained {tr['final']}, equal-time ratio {tr['equal_time_relmse_ratio_plain_over_cache']}; "
+          f"untrained {un['final']}; plain {out['plain_final']}", flush=True)
+    assert tr["train_samples_per_frame"] >= 60000  # ~2^16 training paths' samples per frame
+    assert tr["equal_time_relmse_ratio_plain_over_cache"] > 1.0
+    assert abs(tr["final"]["mean_over_reference"] - 1.0) < 0.02
+    assert un["final"]["mean_over_reference"] < 0.95
+    assert tr["final"]["relmse_8x8"] < 0.5 * un["final"]["relmse_8x8"]
