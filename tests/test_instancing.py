@@ -158,3 +158,38 @@ def test_instance_of_a_missing_mesh_is_refused():
     sc.instances[0] = (5, sc.instances[0][1])
     with pytest.raises(N.VoidrayError, match="instance mesh index out of range"):
         trace_host(sc, np.zeros((1, 3), np.float32), np.float32([[0, 0, 1]]))
+
+
+def test_instance_index_out_of_range_is_refused():
+    sc = _desc_with(None, np.c_[np.eye(3), np.zeros(3)].astype(np.float32))
+    sc.surfaces[0] = N.VR_SURFACE_INSTANCE | 7
+    with pytest.raises(N.VoidrayError, match="instance index out of range"):
+        trace_host(sc, np.zeros((1, 3), np.float32), np.float32([[0, 0, 1]]))
+
+
+def test_a_mesh_placed_directly_and_by_instances():
+    """One mesh both as an ordinary surface (world = object space) and under
+    two instances; an empty mesh under an instance contributes nothing."""
+    from voidray_amd.scene import Mesh
+    sc = Scene.empty()
+    mat = sc.add_material(Materials.lambertian((0.5, 0.5, 0.5)))
+    mi = sc.add_instanced_mesh(W.uv_sphere((0, 0, 0), 1.0, 16, 8))
+    sc.surfaces.append(mi)  # the same mesh as a surface of its own
+    sc.add_object(mat, len(sc.surfaces) - 1)
+    sc.add_object(mat, sc.add_instance(mi, [[0.5, 0, 0, 3.0], [0, 0.5, 0, 0], [0, 0, 0.5, 0]]))
+    sc.add_object(mat, sc.add_instance(mi, [[0, -0.5, 0, -3.0], [0.5, 0, 0, 0], [0, 0, 0.5, 0]]))
+    empty = sc.add_instanced_mesh(Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)))
+    sc.add_object(mat, sc.add_instance(empty, [[1, 0, 0, 0], [0, 1, 0, 5], [0, 0, 1, 0]]))
+    # (off the spheres' vertices and edges: a ray through a shared vertex may
+    # pick another of its triangles in object space)
+    o = np.float32([[0.013, 0.007, -5], [3.013, 0.007, -5], [-2.987, 0.007, -5], [0, 5, -5]])
+    d = np.float32([[0, 0, 1]] * 4)
+    n = sc.meshes[mi].num_triangles
+    for engine in ENGINES:
+        r = trace_host(sc, o, d, engine=engine)
+        prim = r["prim"]
+        assert prim[0] < n and n <= prim[1] < 2 * n and 2 * n <= prim[2] < 3 * n, prim
+        assert prim[3] == 0xFFFFFFFF
+        assert np.allclose(r["t"][:3], [4.0, 4.5, 4.5], atol=1e-2), r["t"]  # (facets sit inside the sphere)
+    flat = sc.flattened()
+    assert np.array_equal(trace_host(flat, o, d)["prim"], trace_host(sc, o, d)["prim"])

@@ -117,3 +117,25 @@ def test_reference_estimator_refuses_instances(gpu_ctx):
     img = np.zeros((4, 4, 4), np.float32)
     with pytest.raises(N.VoidrayError, match="extended estimator only"):
         acc.render(img, make_render_desc(4, 4, 1, 1, RenderSettings(total_samples=1)))
+
+
+def test_instanced_render_with_the_watertight_engine(gpu_ctx):
+    """The watertight engine's instanced kernels in a render (extended
+    estimator, same streams): the image equals the flattened scene's render
+    to far below the Monte Carlo noise."""
+    scene = W.terrain_spheres(grid=24, n_spheres=12, sphere_res=(16, 8), tex_size=64, env_size=(64, 32),
+                              sun_peak=200.0, instanced=True)
+    flat = scene.flattened()
+    settings = RenderSettings(total_samples=16, max_bounces=16, firefly_clamp=float("inf"))
+    desc = make_render_desc(32, 18, 16, 16, settings, seed=3, flags=EXT, rr_depth=4, engine=N.VR_ENGINE_WATERTIGHT)
+    other = make_render_desc(32, 18, 16, 16, settings, seed=4, flags=EXT, rr_depth=4, engine=N.VR_ENGINE_WATERTIGHT)
+    announce("instancing: watertight engine render, instanced vs flattened")
+    imgs = []
+    for sc, dd in ((scene, desc), (flat, desc), (flat, other)):
+        img = np.zeros((18, 32, 4), np.float32)
+        sc.build_acceleration(gpu_ctx).render(img, dd)
+        imgs.append(img)
+    e, noise = relmse(imgs[0], imgs[1]), relmse(imgs[2], imgs[1])
+    print(f"[instancing] watertight render: relMSE vs flattened {e:.3e}, independent-seed noise {noise:.3e}", flush=True)
+    assert np.isfinite(imgs[0]).all()
+    assert e <= 1e-3 * noise
