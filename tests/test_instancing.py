@@ -193,3 +193,21 @@ def test_a_mesh_placed_directly_and_by_instances():
         assert np.allclose(r["t"][:3], [4.0, 4.5, 4.5], atol=1e-2), r["t"]  # (facets sit inside the sphere)
     flat = sc.flattened()
     assert np.array_equal(trace_host(flat, o, d)["prim"], trace_host(sc, o, d)["prim"])
+
+
+def test_tiny_and_huge_instances_are_accepted():
+    """Invertibility does not depend on scale: a sphere scaled by 1e-9 or by
+    1e6 is an instance like any other, and a ray aimed at its centre hits it
+    with the watertight test.  Moller-Trumbore's determinant threshold (t_min,
+    as in the reference) is applied in object space as t_min / |det M|, the
+    same decision as for the placed triangles: the huge sphere is hit, the
+    tiny one is not, in both the instanced and the flattened scene."""
+    d = np.float32([[0, 0, 1]])
+    for s, dist in ((1e-9, 1.0), (1e6, 1e7)):
+        sc = _desc_with(None, np.float32([[s, 0, 0, 0], [0, s, 0, 0], [0, 0, s, 0]]))
+        o = np.float32([[0.013 * s, 0.007 * s, -dist]])
+        assert trace_host(sc, o, d, engine=N.VR_ENGINE_WATERTIGHT)["prim"][0] != 0xFFFFFFFF, s
+        for engine in (N.VR_ENGINE_WIDE, N.VR_ENGINE_REFERENCE):
+            got = trace_host(sc, o, d, engine=engine)["prim"][0]
+            assert got == trace_host(sc.flattened(), o, d, engine=engine)["prim"][0], (s, engine)
+            assert (got != 0xFFFFFFFF) == (s > 1), (s, engine, got)

@@ -80,7 +80,8 @@ struct alignas(16) InstanceDev {
     float nrm[9];          // (M^-1)^T, row-major
     uint32_t mesh, surface, prim_base, blas_root;  // blas_root: QNode index
     float face_sign;       // sign of det M: the transformed winding's face normal is face_sign (M^-1)^T n
-    uint32_t pad[2];
+    float det_inv;         // 1 / |det M|: scales the ray/triangle determinant threshold in object space
+    uint32_t pad;
 };
 static_assert(sizeof(InstanceDev) == 160, "InstanceDev layout");
 constexpr uint32_t kInstanceMarker = 0x7FA1B2C3u;

@@ -243,12 +243,16 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
             const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
                                m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
                                m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-            if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return fail(VR_ERR_INVALID, "singular instance transform");
+            // (scale-free test: an invertible map of any size passes, one
+            // whose inverse does not fit a float does not)
+            bool invertible = det != 0.0 && std::isfinite(det);
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
                     const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
-                    iv[r][c] = (m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1]) / det;
+                    iv[r][c] = invertible ? (m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1]) / det : 0.0;
+                    invertible = invertible && std::isfinite((float)iv[r][c]);
                 }
+            if (!invertible) return fail(VR_ERR_INVALID, "singular instance transform");
             for (int r = 0; r < 3; ++r) {
                 double t = 0;
                 for (int c = 0; c < 3; ++c) {
@@ -259,6 +263,7 @@ int build_host_scene(const vr_scene_desc* d, HostScene& H) {
                 x.inv[4 * r + 3] = (float)t;
             }
             x.face_sign = det < 0 ? -1.0f : 1.0f;
+            x.det_inv = (float)(1.0 / std::fabs(det));
             x.mesh = in.mesh;
             x.surface = i;
             x.prim_base = pb;

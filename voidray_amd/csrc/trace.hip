@@ -166,7 +166,7 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                 if (st.cur == kInstExit) {  // back to the world ray, reloaded
                     wide_instance_exit<WT>(st, mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]),
                                            mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]));
-                } else if (wide_instance_enter<LDS, WT>(a.sv, st, ws, st.cur) == kStepFallback) {
+                } else if (wide_instance_enter<LDS, WT>(a.sv, st, ws, st.cur, a.tmin) == kStepFallback) {
                     fallback = true;
                 }
                 continue;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kWave, 7) void k_trace_persist<false, false, true>(
     trace_persist<false, false, true>(a);
 }
 template <>
-__global__ __launch_bounds__(kWave) void k_trace_persist<true, false, true>(TraceArgs a) {
+__global__ __launch_bounds__(kWave, 6) void k_trace_persist<true, false, true>(TraceArgs a) {
     trace_persist<true, false, true>(a);
 }
 template <>
@@ -348,9 +348,11 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                     // an instance's mesh is walked with the ray in its object space
                     vf3 mo = o, md = d;
                     WTRay mwt = wt;
+                    float meps = -1.0f;
                     if (si.instance >= 0) {
                         instance_ray(sv.instances[si.instance], o, d, mo, md);
                         if (WT) mwt = wt_prepare(md);
+                        meps = a.tmin * sv.instances[si.instance].det_inv;
                     }
                     int32_t tri = -1;
                     if (kind == kCoopMesh) {
@@ -375,7 +377,8 @@ __global__ __launch_bounds__(kWave) void k_trace_exact_wave(TraceArgs a, const u
                         vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
                         vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
                         MTResult r;
-                        if (WT ? wt_hit(mwt, mo, p0, p1, p2, a.tmin, r) : mt_hit(mo, md, p0, p1, p2, a.tmin, r)) {
+                        if (WT ? wt_hit(mwt, mo, p0, p1, p2, a.tmin, r)
+                               : mt_hit(mo, md, p0, p1, p2, a.tmin, r, meps)) {
                             const uint32_t prim = si.prim_base + (uint32_t)tri;
                             if (r.t < bt || (r.t == bt && prim < bprim)) {
                                 bt = r.t;

@@ -218,9 +218,11 @@ struct WideState {
     int sp, lo;
     int top;  // LDS ring slot of entry sp (sp modulo the ring size, kept incrementally)
     bool have, unconfirmed, found;
-    // INST only: the instance being walked (kNoInstance: none) and its prim
-    // base, the instance of the best hit (the world ray is the caller's)
+    // INST only: the instance being walked (kNoInstance: none), its prim
+    // base and triangle determinant threshold, the instance of the best hit
+    // (the world ray is the caller's)
     uint32_t inst, inst_base, best_inst;
+    float inst_eps;
 };
 constexpr uint32_t kNoInstance = 0xFFFFFFFFu;
 // Stack codes of instance entries (INST): bit 30 set, below the leaf bit.
@@ -361,7 +363,9 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, const WideStack& ws, uin
             continue;
         }
         MTResult r;
-        if (WT ? !wt_hit(st.wt, o, p0, p1, p2, tmin, r) : !mt_hit(o, d, p0, p1, p2, tmin, r)) continue;
+        if (WT ? !wt_hit(st.wt, o, p0, p1, p2, tmin, r)
+               : !mt_hit(o, d, p0, p1, p2, tmin, r, in_inst ? st.inst_eps : -1.0f))
+            continue;
         if (ANY) {
             if (r.t < tmax_any) {
                 if (WT || in_inst ||
@@ -402,7 +406,7 @@ VR_HD int wide_leaf(const SceneView& sv, WideState& st, const WideStack& ws, uin
 // when the exit entry does not fit on the stack.  Exit (kInstExit): the caller
 // hands back the world ray (the state does not keep it, to save registers).
 template <int LDS_STACK, bool WT = false>
-VR_HD int wide_instance_enter(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code) {
+VR_HD int wide_instance_enter(const SceneView& sv, WideState& st, const WideStack& ws, uint32_t code, float tmin) {
     const uint32_t k = code & ~kInstEntry;
     if (!wide_push<LDS_STACK>(st, ws, u32x2{kInstExit, 0u})) return kStepFallback;
     const InstanceDev& x = sv.instances[k];
@@ -411,6 +415,7 @@ VR_HD int wide_instance_enter(const SceneView& sv, WideState& st, const WideStac
     wide_set_ray<WT>(st, oo, dd);
     st.inst = k;
     st.inst_base = x.prim_base;
+    st.inst_eps = tmin * x.det_inv;
     st.cur = x.blas_root;
     st.have = true;
     return kStepContinue;
@@ -502,7 +507,7 @@ VR_HD int wide_step(const SceneView& sv, WideState& st, const WideStack& ws, flo
         wide_instance_exit<WT>(st, wo, wd);
         return kStepContinue;
     }
-    if (INST && (cur & kInstEntry)) return wide_instance_enter<LDS_STACK, WT>(sv, st, ws, cur);
+    if (INST && (cur & kInstEntry)) return wide_instance_enter<LDS_STACK, WT>(sv, st, ws, cur, tmin);
     return wide_node<LDS_STACK, ANY>(sv, st, ws, cur, tmin, tmax_any);
 }
 
@@ -611,9 +616,11 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
         // an instance's mesh is walked with the ray in its object space
         vf3 mo = o, md = d;
         WTRay mwt = wt;
+        float meps = -1.0f;
         if (si.instance >= 0) {
             instance_ray(sv.instances[si.instance], o, d, mo, md);
             if (WT) mwt = wt_prepare(md);
+            meps = tmin * sv.instances[si.instance].det_inv;
         }
         auto test = [&](uint32_t tri) {
             const uint32_t* ix = sv.mesh_idx + 3 * (size_t)(mi.tri_base + tri);
@@ -622,7 +629,7 @@ __host__ __device__ inline void trace_exact_stk(const SceneView& sv, vf3 o, vf3 
             vf3 p1 = mk3(vp[3 * ix[1]], vp[3 * ix[1] + 1], vp[3 * ix[1] + 2]);
             vf3 p2 = mk3(vp[3 * ix[2]], vp[3 * ix[2] + 1], vp[3 * ix[2] + 2]);
             MTResult r;
-            if (WT ? !wt_hit(mwt, mo, p0, p1, p2, tmin, r) : !mt_hit(mo, md, p0, p1, p2, tmin, r)) return;
+            if (WT ? !wt_hit(mwt, mo, p0, p1, p2, tmin, r) : !mt_hit(mo, md, p0, p1, p2, tmin, r, meps)) return;
             const uint32_t prim = si.prim_base + tri;
             if (!have || r.t < bt || (r.t == bt && prim < out.prim)) {
                 have = true;

@@ -151,12 +151,17 @@ VR_HD Box3 tri_box_eps(vf3 p0, vf3 p1, vf3 p2) {
 struct MTResult {
     float t, u, v;
 };
-VR_HD bool mt_hit(vf3 o, vf3 d, vf3 p0, vf3 p1, vf3 p2, float tmin, MTResult& r) {
+// det_eps: the determinant threshold, t_min as the reference uses it
+// (negative: t_min).  Inside an instance the triple product a = e1 . (d x e2)
+// is the world one divided by det M, so the instance passes t_min / |det M|
+// to take the same decision as the placed triangle would.
+VR_HD bool mt_hit(vf3 o, vf3 d, vf3 p0, vf3 p1, vf3 p2, float tmin, MTResult& r, float det_eps = -1.0f) {
     vf3 e1 = p1 - p0;
     vf3 e2 = p2 - p0;
     vf3 h = cross3(d, e2);
     float a = dot3(e1, h);
-    if (a > -tmin && a < tmin) return false;
+    const float eps = det_eps < 0.0f ? tmin : det_eps;
+    if (a > -eps && a < eps) return false;
     float f = 1.0f / a;
     vf3 s = o - p0;
     float u = f * dot3(s, h);
