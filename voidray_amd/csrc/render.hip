@@ -297,7 +297,7 @@ __device__ __forceinline__ vf3 fold_levels(const RenderArgs& a, uint32_t p, uint
 // Register budget: with the default bounds (blocks of up to 1024 threads) the
 // compiler holds this kernel at 128 VGPRs, 4 waves per SIMD (64 bytes of
 // scratch per lane).  Measured on the terrain frame against other budgets
-// (before the f64 libm helpers of the microfacet material stopped being
+// (while the microfacet material's transcendentals were still
 // inlined): 233 VGPRs without spills at 2 waves 9% slower, 168 at 3 waves 3%
 // slower, 96 at 5 and 80 at 6 waves (more spills) 7% and 13% slower.
 //

@@ -3,6 +3,7 @@
 // follows the operation order of the reference so that, with the same random
 // words, a path makes the same decisions as voidray's CPU integrator.
 #pragma once
+#include "f32math.h"
 #include "internal.h"
 
 namespace vr {
@@ -40,13 +41,17 @@ VR_HD vf3 bilinear(uint32_t W, uint32_t H, float x, float y, Fetch fetch) {
                  lerp3(fetch(((y0 + 1) * W + x0) % len), fetch(((y0 + 1) * W + x0 + 1) % len), ax), ay);
 }
 
+// Rust's x % 1.0 (fmod(x, 1)): x - trunc(x), which is exact, with the sign
+// of x (fmod(-2, 1) = -0); inf and NaN give NaN.
+VR_HD float fmod1(float x) { return copysignf(x - truncf(x), x); }
+
 // ImageTexture::sample (texture.rs:82-98)
 VR_HD vf3 tex_sample(const SceneView& sv, int32_t ti, float u, float v) {
     const GpuTexture t = sv.textures[ti];
     if (u < 0.0f) u -= truncf(u) - 1.0f;
     if (v < 0.0f) v -= truncf(v) - 1.0f;
-    float x = fmodf(u, 1.0f) * (float)t.w;
-    float y = (1.0f - fmodf(v, 1.0f)) * (float)t.h;
+    float x = fmod1(u) * (float)t.w;
+    float y = (1.0f - fmod1(v)) * (float)t.h;
     if (t.sample_type == VR_SAMPLE_NEAREST) {
         uint64_t xi = as_usize(x), yi = as_usize(y);
         if (xi > t.w - 1) xi = t.w - 1;
@@ -150,23 +155,25 @@ VR_HD vf2 unit_circle(PathRng& rng, uint32_t& capped) {
     return vf2{1.f, 0.f};
 }
 
-// f32 exp, ln, sin, cos, atan as the host's libm gives them (Rust's f32::exp
-// etc. call glibc's expf, logf, ...): evaluated in f64 and rounded once, which
-// agrees with glibc to within one ulp.  The device's own f32 expf returns 0
-// for results in the denormal range (x < about -87.3), where glibc returns the
-// denormal; the Beckmann pdf reaches that range, and 0 there turns a sample
-// the reference keeps (tiny pdf, clamped firefly) into "no scatter".
-// Not inlined: the f64 routines are large, and inlined into the shading
-// kernel their registers made every path spill (344 bytes of scratch per
-// lane; 64 with calls), though only microfacet hits run them.
-__noinline__ VR_HD float exp_libm(float x) { return (float)exp((double)x); }
-__noinline__ VR_HD float ln_libm(float x) { return (float)log((double)x); }
-__noinline__ VR_HD float sin_libm(float x) { return (float)sin((double)x); }
-__noinline__ VR_HD float cos_libm(float x) { return (float)cos((double)x); }
-__noinline__ VR_HD float atan_libm(float x) { return (float)atan((double)x); }
+// f32 exp, ln, atan, sin and cos of the microfacet material (Rust's f32::exp
+// etc., glibc's expf, logf, ... on the reference's platform): the f32-only
+// routines of f32math.h, which return the correctly rounded result but in
+// cases within about 2^-43 relative of a rounding midpoint, denormal results
+// of exp included (the device's own expf returns 0 there, and 0 turns a sample
+// the reference keeps, with a tiny pdf and a clamped firefly, into "no
+// scatter").  Not inlined: only microfacet hits run them, and inlined into the
+// shading kernel their registers would count against every path.
+__noinline__ VR_HD float exp_mf(float x) { return f32m::exp_cr(x); }
+__noinline__ VR_HD float ln_mf(float x) { return f32m::ln_cr(x); }
+__noinline__ VR_HD float atan_mf(float x) { return f32m::atan_cr(x); }
+__noinline__ VR_HD vf2 sincos_mf(float x) {
+    vf2 r;
+    f32m::sincos_cr(x, r.x, r.y);
+    return r;
+}
 
 // Beckmann distribution term D (microfacet.rs:140-143, 183-186).
-VR_HD float beckmann_d(float m2, float nh2) { return exp_libm((nh2 - 1.0f) / (m2 * nh2)) / (((m2 * kPi) * nh2) * nh2); }
+VR_HD float beckmann_d(float m2, float nh2) { return exp_mf((nh2 - 1.0f) / (m2 * nh2)) / (((m2 * kPi) * nh2) * nh2); }
 
 // MicrofacetBSDF::bsdf (microfacet.rs:120-213); n, wo, wi as the caller passes them.
 VR_HD vf3 microfacet_bsdf(const GpuMaterial& m, vf3 n, vf3 wo, vf3 wi) {
@@ -216,7 +223,7 @@ VR_HD vf3 microfacet_bsdf(const GpuMaterial& m, vf3 n, vf3 wo, vf3 wi) {
 VR_HD float beckmann_pdf(float m2, vf3 n, vf3 h) {
     const float cos_t = fabsf(dot3(h, n));
     const float sin_t = sqrtf(1.0f - cos_t * cos_t);
-    return (1.0f / ((kPi * m2) * powi3(cos_t))) * exp_libm((-powi2(sin_t / cos_t)) / m2);
+    return (1.0f / ((kPi * m2) * powi3(cos_t))) * exp_mf((-powi2(sin_t / cos_t)) / m2);
 }
 
 // MicrofacetBSDF::sample (microfacet.rs:215-312); false = None.
@@ -230,8 +237,9 @@ VR_HD bool microfacet_sample(const GpuMaterial& m, vf3 n, vf3 wo, PathRng& rng, 
     const float eta_t = dot3(wo, n) > 0.0f ? m.index : 1.0f / m.index;
     auto beckmann = [&]() {
         // θ = arctan √(-m^2 ln U), azimuth from UnitCircle
-        const float theta = atan_libm(sqrtf(m2 * -ln_libm(rng.uniform())));
-        const float sin_t = sin_libm(theta), cos_t = cos_libm(theta);
+        const float theta = atan_mf(sqrtf(m2 * -ln_mf(rng.uniform())));
+        const vf2 sc = sincos_mf(theta);
+        const float sin_t = sc.x, cos_t = sc.y;
         const vf2 c = unit_circle(rng, capped);
         return local_to_world_mul(n, mk3(c.x * sin_t, c.y * sin_t, cos_t));
     };
