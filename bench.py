@@ -580,6 +580,11 @@ def _serving_leg(ctx, torch, stream, cpu=True):
     }
 
 
+def _progress(msg):
+    """One line on stderr per finished leg (the JSON result stays the one line on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -714,10 +719,14 @@ def main():
 
     mrays = n / (ms * 1e-3) / 1e6
     achieved = BYTES_PER_RAY_CLOSEST * n / (kernel_ms * 1e-3) / 1e9
+    _progress(f"traversal microbench: {mrays:.1f} Mrays/s")
     pmc = _counters_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(scene, o, d)
+    _progress("traversal CPU baseline done")
     renders = None if args.no_render else _render_legs(ctx, cpu=not args.no_cpu)
+    _progress("render legs done")
     serving = None if args.no_serving else _serving_leg(ctx, torch, stream, cpu=not args.no_cpu)
+    _progress("serving leg done")
     cache = None if args.no_cache else _cache_leg(ctx, torch, stream, train_steps=args.cache_train_steps,
                                                   query_roughness=args.cache_query_roughness,
                                                   max_resolution=args.cache_max_res, lr_grid=args.cache_lr[0],
@@ -766,6 +775,7 @@ def main():
             "radiance_cache": cache,
         },
     }
+    _progress("radiance cache leg done")
     print(json.dumps(out), flush=True)
 
 
