@@ -138,16 +138,18 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
         // phase ends once no lane is still looking for its first leaf.  Node
         // and triangle work then run in separate passes instead of splitting
         // every iteration between the two.
-        const bool live = ray != kNone;
+        // The lane's state as one word: 0 walking, 1 done, 2 handed to the
+        // exact engine, 3 no ray.  (As three bools they were three lane
+        // masks in scalar registers, which spilled: 3% slower.)
+        uint32_t status = ray != kNone ? 0u : 3u;
         uint32_t leaf = kNone;
-        bool done = false, fallback = false;
         for (int it = 0; it < kNodeSteps; ++it) {
-            const bool searching = live && !done && !fallback && leaf == kNone;
+            const bool searching = status == 0u && leaf == kNone;
             if (__ballot(searching) == 0ull) break;
-            if (!live || done || fallback) continue;
+            if (status != 0u) continue;
             if (!st.have) {
                 if (!wide_pop_next<LDS, ANY>(st, ws)) {
-                    if (leaf == kNone) done = true;  // stack empty, nothing parked: the ray is finished
+                    if (leaf == kNone) status = 1u;  // stack empty, nothing parked: the ray is finished
                     continue;
                 }
             }
@@ -167,23 +169,23 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
                     wide_instance_exit<WT>(st, mk3(a.o[3 * ray], a.o[3 * ray + 1], a.o[3 * ray + 2]),
                                            mk3(a.d[3 * ray], a.d[3 * ray + 1], a.d[3 * ray + 2]));
                 } else if (wide_instance_enter<LDS, WT>(a.sv, st, ws, st.cur, a.tmin) == kStepFallback) {
-                    fallback = true;
+                    status = 2u;
                 }
                 continue;
             }
             st.have = false;
-            if (wide_node<LDS, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) fallback = true;
+            if (wide_node<LDS, ANY>(a.sv, st, ws, st.cur, a.tmin, a.tmax) == kStepFallback) status = 2u;
         }
         // Leaf phase
-        if (leaf != kNone && !fallback) {
+        if (leaf != kNone && status != 2u) {
             const int rc = wide_leaf<LDS, ANY, WT, INST>(a.sv, st, ws, leaf, a.tmin, a.tmax);
-            if (rc == kStepDone) done = true;
-            else if (rc == kStepFallback) fallback = true;
+            if (rc == kStepDone) status = 1u;
+            else if (rc == kStepFallback) status = 2u;
         }
-        if (live && !done && !fallback && !st.have && st.sp == 0) done = true;
-        if (done || fallback) {
+        if (status == 0u && !st.have && st.sp == 0) status = 1u;
+        if (status == 1u || status == 2u) {
             HitOut h;
-            int rc = fallback ? kStepFallback : wide_finish<ANY, WT, INST>(a.sv, st, a.tmin, h);
+            int rc = status == 2u ? kStepFallback : wide_finish<ANY, WT, INST>(a.sv, st, a.tmin, h);
             if (rc == kStepFallback) {
                 uint32_t slot = atomicAdd(a.fb_count, 1u);
                 a.fb_list[slot] = ray;

@@ -217,7 +217,10 @@ struct WideState {
     uint32_t cur;
     int sp, lo;
     int top;  // LDS ring slot of entry sp (sp modulo the ring size, kept incrementally)
-    bool have, unconfirmed, found;
+    // 0 or 1.  As words they live in vector registers; as bools the compiler
+    // keeps each as a lane mask in scalar registers, which the persistent
+    // kernel runs out of (spilled masks cost readlanes in every node step).
+    uint32_t have, unconfirmed, found;
     // INST only: the instance being walked (kNoInstance: none), its prim
     // base and triangle determinant threshold, the instance of the best hit
     // (the world ray is the caller's)
