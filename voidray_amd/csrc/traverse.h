@@ -450,16 +450,6 @@ VR_HD int wide_node(const SceneView& sv, WideState& st, const WideStack& ws, uin
     // Near and far planes chosen once per node from the ray's direction signs
     // (all four children's bytes at once): with s > 0, fma(lo, s, b) <=
     // fma(hi, s, b), so this equals taking min/max per child and axis.
-    if (!ANY) {
-        // Closest hit: the direction-sign tests below are redone per node.
-        // Hoisted out of the node loop, their lane masks outlive the scalar
-        // registers and come back from VGPR lanes (6 readlanes per node
-        // against 3 compares; measured 0.9% faster; the any-hit kernel,
-        // with fewer registers live, is not).
-        reg_barrier(st.idx);
-        reg_barrier(st.idy);
-        reg_barrier(st.idz);
-    }
     const bool negx = st.idx < 0.f, negy = st.idy < 0.f, negz = st.idz < 0.f;
     uint32_t qnx = negx ? w2.w : w2.x, qfx = negx ? w2.x : w2.w;
     uint32_t qny = negy ? w3.x : w2.y, qfy = negy ? w2.y : w3.x;

@@ -43,15 +43,6 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 VR_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
-
-// An empty register barrier: the compiler must treat v as rewritten at this
-// point, so what is derived from v afterwards is computed here rather than
-// hoisted out of an enclosing loop.  The host build ignores it.
-VR_HD void reg_barrier(float& v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" : "+v"(v));
-#endif
-}
 VR_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 static constexpr float kPi = 3.14159265358979323846f;
