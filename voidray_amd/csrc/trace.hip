@@ -16,7 +16,7 @@ constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 constexpr int kLdsStack = 10;
 constexpr int kLdsStackAny = 8;  // (registers, not LDS, bound the resident waves)
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
-constexpr int kRefill = 32;      // idle lanes that trigger a wave-wide refill (closest hit; 16, 24 and 40 measured slower)
+constexpr int kRefill = 40;      // idle lanes that trigger a wave-wide refill (closest hit; 24, 32, 48 and 56 measured slower)
 constexpr int kRefillAny = 40;   // the same for any-hit queries (shorter rays: refill less often)
 constexpr int kNodeSteps = 4;    // most node-phase iterations between two leaf phases
 
@@ -83,8 +83,8 @@ static_assert(kFetchParts <= 64, "one lane clears each partition counter");
 // (dynamic ray fetch), so long incoherent rays do not leave the wave idle.
 // A refill costs an atomic round trip plus the new rays' loads, and with a
 // spatial-split BVH a ray needs only ~20 steps, so refills wait until a
-// good part of the wave is idle (24 of 64 lanes; 40 for the shorter any-hit
-// rays) and draw from partitioned counters (below).
+// good part of the wave is idle (40 of 64 lanes, for closest and any hit)
+// and draw from partitioned counters (below).
 template <bool ANY, bool WT, bool INST>
 __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
     constexpr int LDS = ANY ? kLdsStackAny : kLdsStack;
