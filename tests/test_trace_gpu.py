@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from tests.conftest import announce
-from tests.trace_cases import compare_hits, edge_case_scene
+from tests.trace_cases import compare_hits, default_box_scene, edge_case_scene
 from voidray_amd import _native as N
 from voidray_amd import workloads as W
 from voidray_amd.scene import Scene
@@ -90,6 +90,20 @@ def test_edge_cases(gpu_ctx, oracle_lib):
     acc = scene.build_acceleration(gpu_ctx)
     _compare(ref, acc.hit(o, d), "edges/wide")
     _compare(ref, acc.hit(o, d, engine=N.VR_ENGINE_REFERENCE), "edges/exact")
+
+
+def test_default_boxes_beside_flat_boxes(gpu_ctx, oracle_lib):
+    """Empty surfaces (default boxes, which AABB::hit passes unconditionally)
+    next to flat sibling boxes (which it rejects): both engines equal the
+    oracle bit for bit.  The default-box test once compiled to a
+    wave-uniform value (one lane's answer for all), which lanes testing the
+    two kinds of box in one wave would get wrong."""
+    scene, o, d = default_box_scene()
+    ref = oracle_lib.OracleScene(scene).trace(o, d)
+    acc = scene.build_acceleration(gpu_ctx)
+    for engine in (N.VR_ENGINE_WIDE, N.VR_ENGINE_REFERENCE):
+        announce(f"trace default boxes beside flat boxes, engine {engine}, n={len(o)}")
+        _compare(ref, acc.hit(o, d, engine=engine), f"default-boxes/{engine}")
 
 
 def test_empty_scene_and_zero_rays(gpu_ctx):

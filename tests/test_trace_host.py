@@ -9,7 +9,7 @@ Triangle::hit, core/mesh.rs:144-188).
 """
 import numpy as np
 
-from tests.trace_cases import compare_hits, edge_case_scene
+from tests.trace_cases import compare_hits, default_box_scene, edge_case_scene
 from voidray_amd import _native as N
 from voidray_amd import workloads as W
 from voidray_amd.device import trace_host
@@ -64,6 +64,11 @@ def test_multi_surface_scene(oracle_lib):
 def test_edge_cases(oracle_lib):
     scene, o, d = edge_case_scene()
     _check_all(scene, o, d, oracle_lib, "edges")
+
+
+def test_default_boxes_beside_flat_boxes(oracle_lib):
+    scene, o, d = default_box_scene()
+    _check_all(scene, o, d, oracle_lib, "default-boxes")
 
 
 def test_golden_fixture():

@@ -36,6 +36,37 @@ def edge_case_scene():
     return scene, o, d
 
 
+def default_box_scene():
+    """Empty surfaces (AABB::default() boxes in the top-level tree) beside
+    axis-aligned quads in the plane z = 0.5 (siblings with a flat union box,
+    which the reference's slab test rejects for every ray not in the plane:
+    aabb.rs:109,126,143) and a 3,000-triangle soup.  Rays: straight up and
+    down through the quads, plus random ones.  Within one wave of the exact
+    engine, lanes then test default boxes and flat boxes side by side.
+    Returns (scene, origins, directions)."""
+    from voidray_amd import workloads as W
+    f32 = np.float32
+    sc = Scene.empty()
+    m = sc.add_material(Materials.lambertian((0.5, 0.5, 0.5)))
+    empty = Mesh(np.zeros((0, 3), f32), np.zeros((0, 3), np.uint32))
+    for k in range(12):
+        if k % 3 == 0:
+            sc.add_object(m, sc.add_mesh(empty))
+        x0 = f32(k * 0.25)
+        pos = np.array([[x0, 0, 0.5], [x0 + 0.2, 0, 0.5], [x0 + 0.2, 1, 0.5], [x0, 1, 0.5]], f32)
+        sc.add_object(m, sc.add_mesh(Mesh(pos, np.array([[0, 1, 2], [0, 2, 3]], np.uint32))))
+    sc.add_object(m, sc.add_mesh(W.random_triangle_scene(3000, seed=13).meshes[0]))
+    g = np.linspace(0.01, 2.99, 96, dtype=f32)
+    h = np.linspace(0.01, 0.99, 32, dtype=f32)
+    pts = np.stack(np.meshgrid(g, h), -1).reshape(-1, 2)
+    rng = np.random.default_rng(5)
+    o = np.concatenate([np.c_[pts, np.zeros(len(pts))], np.c_[pts, np.ones(len(pts))],
+                        rng.uniform(0, 1, (4096, 3)) * [3, 1, 1]]).astype(f32)
+    d = np.concatenate([np.tile([0, 0, 1], (len(pts), 1)), np.tile([0, 0, -1], (len(pts), 1)),
+                        W._normalize_rows(rng.normal(size=(4096, 3)))]).astype(f32)
+    return sc, o, d
+
+
 def compare_hits(ref, got, label):
     """Bit-exact: prim everywhere, t/u/v wherever the reference hits."""
     miss = ref["prim"] == 0xFFFFFFFF
