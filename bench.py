@@ -157,9 +157,16 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     ref_ms = e0.elapsed_time(e1)
     ref = img[..., :3].double().cpu().numpy()
 
+    def box8(x):
+        hh, ww = x.shape[0] // 8 * 8, x.shape[1] // 8 * 8
+        return x[:hh, :ww, :3].reshape(hh // 8, 8, ww // 8, 8, 3).mean((1, 3))
+
+    ref8 = box8(ref)
+
     def curve(passes, run_pass, total):
         """Runs `passes` passes with an image snapshot after each; returns
-        [(cumulative device ms, spp so far, relMSE of the snapshot rescaled to its spp)]."""
+        [(cumulative device ms, spp so far, relMSE of the snapshot rescaled to
+        its spp, relMSE of its 8x8-pixel box means)]."""
         img.zero_()
         torch.cuda.synchronize()
         spans, snaps = [], []
@@ -172,8 +179,12 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
             snaps.append((done, img.clone() if done else None))
         torch.cuda.synchronize()
         cum = np.cumsum([a.elapsed_time(b) for a, b in spans])
-        return [(float(cum[k]), d, relmse(s.cpu().numpy() * (total / d), ref))
-                for k, (d, s) in enumerate(snaps) if d]
+        out = []
+        for k, (d, s) in enumerate(snaps):
+            if d:
+                x = s.cpu().numpy()[..., :3].astype(np.float64) * (total / d)
+                out.append((float(cum[k]), d, relmse(x, ref), relmse(box8(x), ref8)))
+        return out
 
     spf = 4
 
@@ -186,17 +197,13 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
     plain_img = img[..., :3].double().cpu().numpy()
     out = {"scene": f"terrain {w}x{h}, {scene.num_triangles} triangles", "reference_spp": ref_spp,
            "reference_ms": round(ref_ms, 1),
-           "plain": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5)} for t, s, e in plain_pts[::4] + plain_pts[-1:]]}
+           "plain": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5), "relmse_8x8": round(e8, 6)}
+                     for t, s, e, e8 in plain_pts[::4] + plain_pts[-1:]]}
     lo, hi = scene_bounds(scene)
     warm, frames = warm_frames, 32
     ww, wh, wspp = warm_size
     scratch = torch.zeros_like(img)
 
-    def box8(x):
-        hh, ww = x.shape[0] // 8 * 8, x.shape[1] // 8 * 8
-        return x[:hh, :ww, :3].reshape(hh // 8, 8, ww // 8, 8, 3).mean((1, 3))
-
-    ref8 = box8(ref)
     for depth in depths:
         for trained in (True, False):
             cache = RadianceCache(ctx, lo, hi, log2_table=19, base_resolution=16, max_resolution=max_resolution,
@@ -231,23 +238,33 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
             final = img[..., :3].double().cpu().numpy()
             # plain relMSE at the cached run's final time: log-log interpolation of the plain curve
             pt = np.log([p[0] for p in plain_pts])
-            pe = np.log([p[2] for p in plain_pts])
-            t_end, _, e_end = pts[-1]
-            plain_at = float(np.exp(np.interp(np.log(t_end), pt, pe, left=np.nan, right=np.nan)))
-            if not np.isfinite(plain_at):  # beyond the plain curve: 1/t extrapolation from its last point
-                plain_at = plain_pts[-1][2] * plain_pts[-1][0] / t_end
-            # time the cached run needs to reach the plain tracer's final relMSE
-            target_t, target_e = plain_pts[-1][0], plain_pts[-1][2]
-            reach = next((t for t, _, e in pts if e <= target_e), None)
+            t_end, _, e_end, e8_end = pts[-1]
+
+            def plain_at(col):
+                """Plain tracer's error (column col) at the cached run's final
+                time: log-log interpolation, 1/t beyond its last point."""
+                v = float(np.exp(np.interp(np.log(t_end), pt, np.log([p[col] for p in plain_pts]),
+                                           left=np.nan, right=np.nan)))
+                return v if np.isfinite(v) else plain_pts[-1][col] * plain_pts[-1][0] / t_end
+
+            # time the cached run needs to reach the plain tracer's final error
+            target_t = plain_pts[-1][0]
+            reach = next((p[0] for p in pts if p[2] <= plain_pts[-1][2]), None)
+            reach8 = next((p[0] for p in pts if p[3] <= plain_pts[-1][3]), None)
             out[f"cache_depth_{depth}_{'trained' if trained else 'untrained'}"] = {
-                "points": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5)} for t, s, e in pts[::4] + pts[-1:]],
+                "points": [{"ms": round(t, 2), "spp": s, "relmse": round(e, 5), "relmse_8x8": round(e8, 6)}
+                           for t, s, e, e8 in pts[::4] + pts[-1:]],
                 "final": {"ms": round(t_end, 2), "relmse": round(e_end, 5),
                           "mean_over_reference": round(float(final.mean() / ref.mean()), 4),
                           "relmse_8x8": round(relmse(box8(final), ref8), 6)},
                 "train_samples_per_frame": stats["train_samples"], "queries_per_frame": stats["queries"],
                 "train_ms_per_frame": round(stats["train_ms"], 3), "render_ms_per_frame": round(stats["render_ms"], 3),
-                "equal_time_relmse_ratio_plain_over_cache": round(plain_at / e_end, 3),
-                "time_to_reach_plain_final_relmse_ratio": (round(reach / target_t, 3) if reach else None)}
+                "equal_time_relmse_ratio_plain_over_cache": round(plain_at(2) / e_end, 3),
+                "time_to_reach_plain_final_relmse_ratio": (round(reach / target_t, 3) if reach else None),
+                # the same on 8x8-pixel box means: low-frequency error, which a
+                # biased (dark) cache cannot average away
+                "equal_time_relmse_8x8_ratio_plain_over_cache": round(plain_at(3) / e8_end, 3),
+                "time_to_reach_plain_final_relmse_8x8_ratio": (round(reach8 / target_t, 3) if reach8 else None)}
     out["plain_final"] = {"ms": round(plain_pts[-1][0], 2), "relmse": round(plain_pts[-1][2], 5),
                           "mean_over_reference": round(float(plain_img.mean() / ref.mean()), 4),
                           "relmse_8x8": round(relmse(box8(plain_img), ref8), 6)}
