@@ -684,14 +684,14 @@ def main():
     torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
 
-    def step(lo=0, cnt=n, any_hit=False):
+    def step(lo=0, cnt=n, any_hit=False, timing=False):
         if any_hit:
             acc.trace_device(to.data_ptr() + 12 * lo, td.data_ptr() + 12 * lo, cnt, any_hit=True,
                              d_hit=thit.data_ptr() + lo, stream=sp)
         else:
             acc.trace_device(to.data_ptr() + 12 * lo, td.data_ptr() + 12 * lo, cnt, d_prim=prim.data_ptr() + 4 * lo,
                              d_t=tt.data_ptr() + 4 * lo, d_u=tu.data_ptr() + 4 * lo, d_v=tv.data_ptr() + 4 * lo,
-                             stream=sp)
+                             stream=sp, timing=timing)
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -725,7 +725,7 @@ def main():
     # duration of the dominant kernel alone (HIP events the library records around it)
     kern = []
     for _ in range(args.steps):
-        step()
+        step(timing=True)
         kern.append(acc.last_wide_ms())
     kernel_ms = sum(kern) / len(kern)
     half = n // 2

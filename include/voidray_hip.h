@@ -229,6 +229,10 @@ int vr_scene_get_stats(const vr_scene* scene, vr_scene_stats* out);
 
 #define VR_TRACE_CLOSEST 0
 #define VR_TRACE_ANY 1
+/* OR-ed into vr_trace_desc.mode: record HIP events around the wide-engine
+ * kernel so that vr_trace_last_timing can report its device time (two event
+ * records per call, a few microseconds; off by default). */
+#define VR_TRACE_TIMING 0x100
 /* traversal engines */
 #define VR_ENGINE_WIDE 0      /* quantised BVH4 + robust-winner check + exact fallback */
 #define VR_ENGINE_REFERENCE 1 /* flattened reference trees, both children, no pruning  */
@@ -243,7 +247,7 @@ int vr_scene_get_stats(const vr_scene* scene, vr_scene_stats* out);
 typedef struct vr_trace_desc {
     uint32_t num_rays;
     float t_min;          /* SceneAcceleration uses 0.00001 (scene.rs:183) */
-    int32_t mode;         /* VR_TRACE_* */
+    int32_t mode;         /* VR_TRACE_CLOSEST or VR_TRACE_ANY, optionally | VR_TRACE_TIMING */
     int32_t engine;       /* VR_ENGINE_* */
 } vr_trace_desc;
 
@@ -303,7 +307,8 @@ int vr_trace_host(const vr_scene_desc* scene, const vr_trace_desc* desc,
  * the stream of that trace has been synchronised). */
 int vr_trace_last_fallbacks(vr_scene* scene, uint32_t* count);
 /* Device time (ms, HIP events on the trace's stream) of the wide-engine kernel
- * of the last vr_trace / vr_trace_device_async; waits for that kernel. */
+ * of the last vr_trace / vr_trace_device_async, which must have carried
+ * VR_TRACE_TIMING in its mode; waits for that kernel. */
 int vr_trace_last_timing(vr_scene* scene, float* wide_ms);
 
 /* ---- rendering (render/iterative.rs:11-55, core/tracer.rs:9-56) ------- */

@@ -204,6 +204,7 @@ pub const VR_NOALLOC_MAX_RAYS: u32 = 1_048_576;
 pub const VR_NOALLOC_SCRATCH_BYTES: u32 = 75_501_568;
 pub const VR_TRACE_CLOSEST: i32 = 0;
 pub const VR_TRACE_ANY: i32 = 1;
+pub const VR_TRACE_TIMING: i32 = 0x100;
 pub const VR_ENGINE_WIDE: i32 = 0;
 pub const VR_ENGINE_REFERENCE: i32 = 1;
 pub const VR_ENGINE_WATERTIGHT: i32 = 2;

@@ -19,7 +19,7 @@ VR_BUILD_DEFAULT, VR_BUILD_NO_ENV_CDF, VR_BUILD_LEAF_RECORDS, VR_BUILD_LEAF_REFS
 VR_ANALYTIC_SPHERE, VR_ANALYTIC_GROUND_PLANE = 0, 1
 VR_SURFACE_ANALYTIC = 0x80000000
 VR_SURFACE_INSTANCE = 0x40000000
-VR_TRACE_CLOSEST, VR_TRACE_ANY = 0, 1
+VR_TRACE_CLOSEST, VR_TRACE_ANY, VR_TRACE_TIMING = 0, 1, 0x100
 VR_ENGINE_WIDE, VR_ENGINE_REFERENCE, VR_ENGINE_WATERTIGHT = 0, 1, 2
 VR_NOALLOC_MAX_RAYS = 1048576
 VR_NOALLOC_SCRATCH_BYTES = 75501568

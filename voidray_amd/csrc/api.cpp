@@ -178,7 +178,8 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
                   uint32_t* fallbacks) {
     return guarded([&] {
         if (!sd || !desc) return fail(VR_ERR_INVALID, "null argument");
-        if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
+        const int32_t mode = desc->mode & ~VR_TRACE_TIMING;  // timing: device calls only
+        if (mode != VR_TRACE_CLOSEST && mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
         if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE &&
             desc->engine != VR_ENGINE_WATERTIGHT)
             return fail(VR_ERR_INVALID, "bad engine");
@@ -192,7 +193,7 @@ int vr_trace_host(const vr_scene_desc* sd, const vr_trace_desc* desc, const floa
             stats->num_tri_records = H.stats.num_tri_records;
         }
         const vr::SceneView& sv = H.view;
-        const bool any = desc->mode == VR_TRACE_ANY;
+        const bool any = mode == VR_TRACE_ANY;
         constexpr int kStack = 16, kSpill = 240;
         u32x2 lds[kStack], spill[kSpill];
         const vr::WideStack ws{lds, 1, spill, 1, kSpill};
@@ -238,7 +239,8 @@ int vr_trace_last_fallbacks(vr_scene* s, uint32_t* count) {
 
 int vr_trace_last_timing(vr_scene* s, float* wide_ms) {
     if (!s || !wide_ms) return fail(VR_ERR_INVALID, "null argument");
-    if (!s->trace_timed || !s->trace_ev[1]) return fail(VR_ERR_INVALID, "no timed wide-engine trace yet");
+    if (!s->trace_timed || !s->trace_ev[1])
+        return fail(VR_ERR_INVALID, "the last trace was not a timed wide-engine trace (mode | VR_TRACE_TIMING)");
     VR_HIP_CHECK(hipEventSynchronize(s->trace_ev[1]));
     VR_HIP_CHECK(hipEventElapsedTime(wide_ms, s->trace_ev[0], s->trace_ev[1]));
     return 0;

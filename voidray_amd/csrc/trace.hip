@@ -575,7 +575,8 @@ int launch_trace(vr_scene* s, const TraceJob& j, hipStream_t stream) {
 
 static int check_trace_desc(const vr_trace_desc* desc, const float* d_o, const float* d_d) {
     if (!d_o || !d_d) return fail(VR_ERR_INVALID, "null ray buffers");
-    if (desc->mode != VR_TRACE_CLOSEST && desc->mode != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
+    const int32_t m = desc->mode & ~VR_TRACE_TIMING;
+    if (m != VR_TRACE_CLOSEST && m != VR_TRACE_ANY) return fail(VR_ERR_INVALID, "bad trace mode");
     if (desc->engine != VR_ENGINE_WIDE && desc->engine != VR_ENGINE_REFERENCE && desc->engine != VR_ENGINE_WATERTIGHT)
         return fail(VR_ERR_INVALID, "bad engine");
     return 0;
@@ -593,18 +594,18 @@ int trace_device(vr_scene* s, const vr_trace_desc* desc, const float* d_o, const
     j.n_max = n;
     j.tmin = desc->t_min;
     j.tmax = kInf;
-    j.any = desc->mode == VR_TRACE_ANY;
+    j.any = (desc->mode & ~VR_TRACE_TIMING) == VR_TRACE_ANY;
     j.engine = desc->engine;
     j.prim = d_prim;
     j.t = d_t;
     j.u = d_u;
     j.v = d_v;
     j.hit = d_hit;
-    if (!s->trace_ev[0]) {
+    j.timed = (desc->mode & VR_TRACE_TIMING) && desc->engine != VR_ENGINE_REFERENCE;
+    if (j.timed && !s->trace_ev[0]) {
         VR_HIP_CHECK(hipEventCreate(&s->trace_ev[0]));
         VR_HIP_CHECK(hipEventCreate(&s->trace_ev[1]));
     }
-    j.timed = desc->engine != VR_ENGINE_REFERENCE;
     s->trace_timed = j.timed;
     rc = launch_trace(s, j, stream);
     if (rc) return rc;
@@ -696,7 +697,7 @@ int trace_device_noalloc(vr_scene* s, const vr_trace_desc* desc, const float* d_
     j.n_max = n;
     j.tmin = desc->t_min;
     j.tmax = kInf;
-    j.any = desc->mode == VR_TRACE_ANY;
+    j.any = (desc->mode & ~VR_TRACE_TIMING) == VR_TRACE_ANY;
     j.engine = desc->engine;
     j.prim = d_prim;
     j.t = d_t;

@@ -88,15 +88,18 @@ class SceneAcceleration:
         return c.value
 
     def last_wide_ms(self):
-        """Device time of the wide-engine kernel of the last trace (HIP events)."""
+        """Device time of the wide-engine kernel of the last trace (HIP events;
+        that trace must have been made with timing=True)."""
         ms = C.c_float()
         N.check(N.lib().vr_trace_last_timing(self.handle, C.byref(ms)))
         return ms.value
 
     def trace_device(self, d_origins, d_directions, n, any_hit=False, t_min=1e-5, engine=N.VR_ENGINE_WIDE,
-                     d_prim=None, d_t=None, d_u=None, d_v=None, d_hit=None, stream=None):
-        """Device pointers (ints) in/out, enqueued on `stream` (None = context stream)."""
-        desc = N.vr_trace_desc(n, t_min, N.VR_TRACE_ANY if any_hit else N.VR_TRACE_CLOSEST, engine)
+                     d_prim=None, d_t=None, d_u=None, d_v=None, d_hit=None, stream=None, timing=False):
+        """Device pointers (ints) in/out, enqueued on `stream` (None = context stream).
+        timing: record the wide-engine kernel's device time for last_wide_ms()."""
+        mode = (N.VR_TRACE_ANY if any_hit else N.VR_TRACE_CLOSEST) | (N.VR_TRACE_TIMING if timing else 0)
+        desc = N.vr_trace_desc(n, t_min, mode, engine)
         N.check(N.lib().vr_trace_device_async(self.handle, C.byref(desc), d_origins, d_directions, d_prim, d_t,
                                               d_u, d_v, d_hit, stream))
 
