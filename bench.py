@@ -108,7 +108,7 @@ def _terrain(ctx):
 
 def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_steps=8, warm_frames=0,
                warm_size=(240, 135, 2), query_roughness=0.3, max_resolution=1024, lr_grid=1e-2, lr_mlp=1e-3,
-               depths=(2, 1)):
+               depths=(2, 1), train_every=1):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
     + MIS) at width x height, judged against the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
@@ -224,11 +224,12 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
                                        train_steps=train_steps if trained else 0, frame=k, stream=sp,
                                        query_roughness=query_roughness)
                     return 0
+                train = (k - warm) % train_every == 0  # frames between training frames only query
                 cache.render_frame(accel, img.data_ptr(), make_render_desc(w, h, spf, frames * spf, st,
                                                                            sample_offset=(k - warm) * spf, seed=7,
                                                                            flags=ext, rr_depth=4),
-                                   train_paths=1 << 16, train_vertices=4, cache_depth=depth,
-                                   train_steps=train_steps if trained else 0, frame=k, stream=sp,
+                                   train_paths=(1 << 16) if train else 0, train_vertices=4, cache_depth=depth,
+                                   train_steps=train_steps if trained and train else 0, frame=k, stream=sp,
                                    query_roughness=query_roughness)
                 return spf
 
@@ -270,7 +271,7 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
                           "relmse_8x8": round(relmse(box8(plain_img), ref8), 6)}
     out["query_roughness"] = query_roughness
     out["grid"] = {"log2_table": 19, "base_resolution": 16, "max_resolution": max_resolution, "lr_grid": lr_grid,
-                   "lr_mlp": lr_mlp, "warm_frames": warm_frames}
+                   "lr_mlp": lr_mlp, "warm_frames": warm_frames, "train_every": train_every}
     out["roofline"] = _cache_roofline(ctx, torch, stream, lo, hi)
     return out
 
@@ -625,6 +626,7 @@ def main():
                     help="Adam learning rates of the hash grid and the MLP")
     ap.add_argument("--cache-depths", default="2,1", help="query depths to run (comma-separated)")
     ap.add_argument("--cache-warm", type=int, default=0, help="training-only warm-up frames")
+    ap.add_argument("--cache-train-every", type=int, default=1, help="train on every n-th rendered frame")
     ap.add_argument("--nrc-probe", action="store_true",
                     help="the learned cache's kernels alone (for profiler runs): inference over 2^20 queries and "
                          "training steps over 2^16 samples, 11 launches each, timed with HIP events; prints one line")
@@ -652,7 +654,8 @@ def main():
                                                        query_roughness=args.cache_query_roughness,
                                                        max_resolution=args.cache_max_res, lr_grid=args.cache_lr[0],
                                                        lr_mlp=args.cache_lr[1], warm_frames=args.cache_warm,
-                                                       depths=tuple(int(x) for x in args.cache_depths.split(",")))}),
+                                                       depths=tuple(int(x) for x in args.cache_depths.split(",")),
+                                                       train_every=args.cache_train_every)}),
               flush=True)
         return
     if args.nrc_probe:
@@ -748,7 +751,8 @@ def main():
                                                   query_roughness=args.cache_query_roughness,
                                                   max_resolution=args.cache_max_res, lr_grid=args.cache_lr[0],
                                                   lr_mlp=args.cache_lr[1], warm_frames=args.cache_warm,
-                                                  depths=tuple(int(x) for x in args.cache_depths.split(",")))
+                                                  depths=tuple(int(x) for x in args.cache_depths.split(",")),
+                                                  train_every=args.cache_train_every)
     out = {
         "metric": "Mrays/s closest-hit (traversal microbench: 1M random tris, 4M rays)",
         "value": round(mrays, 2),
