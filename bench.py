@@ -533,7 +533,7 @@ def _serving_leg(ctx, torch, stream, cpu=True):
     # gain on the real batch: the proxy's pick and the same block at the
     # largest chunk are each run once on the real batch, and the faster one kept
     confirm = {}
-    for cand in dict.fromkeys([(chunk, block), (1 << 26, block)]):
+    for cand in dict.fromkeys([(chunk, block), (1 << 27, block)]):
         br.submit(batch, tuning=cand, graph=True, shortest_first=True, stream=sp)
         br.stats()
         c0 = torch.cuda.Event(enable_timing=True)

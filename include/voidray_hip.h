@@ -377,8 +377,9 @@ typedef struct vr_request {
 } vr_request;
 
 typedef struct vr_tuning {
-    uint32_t chunk_paths;     /* paths per wavefront chunk (queue length); 0 = 2^26, halved
-                                 while the batch's path state would exceed 24 GiB */
+    uint32_t chunk_paths;     /* paths per wavefront chunk (queue length, at most 2^27);
+                                 0 = 2^27, halved while the batch's path state would
+                                 exceed 48 GiB */
     uint32_t block;           /* workgroup size of raygen/shade/accumulate; 0 = 256 */
 } vr_tuning;
 

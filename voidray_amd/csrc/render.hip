@@ -40,16 +40,18 @@ namespace vr {
 // Paths per wavefront chunk.  Every bounce of a chunk is a few dependent
 // launches over the chunk's live paths; once Russian roulette and escapes
 // thin them out, a small chunk leaves the traversal kernels' persistent waves
-// mostly idle.  On the 1080p x 64 spp terrain frame: 2^22 paths per chunk
-// 394 ms, 2^24 215 ms, 2^26 166 ms (about 133 B of path state per path with
-// the extended estimator, 272 B with the reference estimator's 16-level
-// attenuation stack: 9 or 18 GB of the GPU's 288 GB at 2^26).
-constexpr uint32_t kDefaultChunkPaths = 1u << 26;
-constexpr uint32_t kMaxChunkPaths = 1u << 26;
+// mostly idle.  On the 1080p x 64 spp terrain frame (132.7 M paths): 2^22
+// paths per chunk 394 ms, 2^24 215 ms, 2^26 (two chunks) 146 ms, 2^27 (one
+// chunk) 137 ms; about 133 B of path state per path with the extended
+// estimator, 272 B with the reference estimator's 16-level attenuation stack:
+// 18 or 36 GB of the GPU's 288 GB at 2^27.  Scratch is sized by the largest
+// chunk a call actually uses, so small frames allocate little.
+constexpr uint32_t kDefaultChunkPaths = 1u << 27;
+constexpr uint32_t kMaxChunkPaths = 1u << 27;
 // The default chunk is halved until the batch's path state fits this budget
 // (the reference estimator's attenuation stack grows with max_bounces: at the
-// 64-bounce limit a 2^26-path chunk would hold 55 GB).
-constexpr size_t kDefaultScratchBudget = 24ull << 30;
+// 64-bounce limit a 2^27-path chunk would hold 110 GB).
+constexpr size_t kDefaultScratchBudget = 48ull << 30;
 constexpr uint32_t kMaxBouncesLimit = 64;
 constexpr uint32_t kShadeGrid = 2048;  // most blocks a grid-stride kernel over a chunk launches
 
@@ -718,7 +720,7 @@ static int plan_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, const vr
                       hipStream_t st, const CacheReq* creq) {
     RenderState& R = *ctx->render;
     uint32_t chunk_paths = tune.chunk_paths ? tune.chunk_paths : kDefaultChunkPaths;
-    if (chunk_paths > kMaxChunkPaths) return fail(VR_ERR_INVALID, "chunk_paths above 2^26");
+    if (chunk_paths > kMaxChunkPaths) return fail(VR_ERR_INVALID, "chunk_paths above 2^27");
     uint32_t block = tune.block ? tune.block : 256;
     if (block % 64 != 0 || block > 1024) return fail(VR_ERR_INVALID, "block must be a multiple of 64 up to 1024");
     drop_graph(R);
@@ -1193,7 +1195,7 @@ int batch_latencies(vr_ctx* ctx, double* ms, uint32_t n) {
 // relaunches after one warm-up launch) and returns the fastest.
 int autotune_batch(vr_ctx* ctx, const vr_request* items, uint32_t n, uint32_t flags, uint32_t reps,
                    hipStream_t st, vr_tuning* best, float* best_ms) {
-    static const uint32_t chunks[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
+    static const uint32_t chunks[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27};
     static const uint32_t blocks[] = {128, 256, 512};
     uint32_t max_spp = 1;
     for (uint32_t i = 0; i < n; ++i) max_spp = std::max(max_spp, items[i].desc.samples);
