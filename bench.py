@@ -34,8 +34,8 @@ def _counters_from_profiles():
         return json.load(f)
 
 
-# Host threads for the CPU baselines: a one-GPU box grants its job 16 host
-# cores (a share of the node's), whatever the node's logical CPU count says, so
+# Host threads for the CPU baselines: a one-GPU job is granted 16 host cores
+# (a share of the node's), whatever the node's logical CPU count says, so
 # the CPU legs use at most 16 threads; the node's count and CPU model are
 # recorded beside each baseline.
 CPU_THREADS = 16
@@ -110,7 +110,7 @@ def _cache_leg(ctx, torch, stream, width=960, height=540, ref_spp=4096, train_st
                warm_size=(240, 135, 2), query_roughness=0.3, max_resolution=1024, lr_grid=1e-2, lr_mlp=1e-3,
                depths=(2, 1), train_every=1):
     """Learned radiance cache on the terrain scene (depth 16, RR from 4, env IS
-    + MIS) at width x height, judged against the plain tracer at ref_spp by
+    + MIS) at width x height, compared with the plain tracer at ref_spp by
     relMSE = mean((x - r)^2 / (r^2 + 0.01)) at equal device time.
     Plain: progressive 4-spp passes, image snapshot after each.  Cached:
     `warm_frames` warm-up frames that only train (the same camera at
@@ -635,8 +635,8 @@ def main():
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE); prints nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this implementation runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is not built)"}))
         sys.exit(2)
 
     import numpy as np
