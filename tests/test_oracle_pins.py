@@ -12,7 +12,7 @@ Lambertian::scatter (simple.rs:102-132), LambertianBSDF with the BSDFMaterial
 blanket impl (simple.rs:60-81, core/traits.rs:19-41), Emission::scatter
 (simple.rs:163-185), CameraAcceleration::cast_ray (core/camera.rs:69-82),
 the pixel -> NDC map of iterative.rs:25-33, AABB::hit (util/aabb.rs:86-148),
-rand_distr's UnitDisc, and this build's environment importance sampler (no
+rand_distr's UnitDisc, and the library's environment importance sampler (no
 reference counterpart).
 """
 import math
