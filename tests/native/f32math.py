@@ -20,6 +20,8 @@ def lib():
         for name in ("f32m_eval_host", "f32m_eval_libm", "f32m_eval_f64", "f32m_eval_device"):
             getattr(L, name).argtypes = [ctypes.c_int, P, P, U64]
             getattr(L, name).restype = ctypes.c_int
+        L.f32m_pow.argtypes = [ctypes.c_int, P, P, P, U64]
+        L.f32m_pow.restype = ctypes.c_int
         L.f32m_sweep.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, P, P, U64]
         L.f32m_sweep.restype = ctypes.c_int
         _lib = L
@@ -35,6 +37,18 @@ def evaluate(where, fn, x):
     if rc != 0:
         raise RuntimeError(f"f32m_eval_{where}({fn}) returned {rc}")
     return y
+
+
+def power(where, x, y):
+    """pow over pairs; where: "host", "device", "f64" or "libm"."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float32), x.shape))
+    z = np.empty_like(x)
+    rc = lib().f32m_pow({"host": 0, "device": 1, "f64": 2, "libm": 3}[where], x.ctypes.data, y.ctypes.data,
+                        z.ctypes.data, len(x))
+    if rc != 0:
+        raise RuntimeError(f"f32m_pow({where}) returned {rc}")
+    return z
 
 
 def sweep(fn, lo, hi, threads=None, cap=4096):

@@ -30,6 +30,11 @@ __host__ __device__ float eval_cr(int fn, float x) {
     }
 }
 
+__global__ void k_pow(const float* x, const float* y, float* z, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = vr::f32m::pow_cr(x[i], y[i]);
+}
+
 __global__ void k_eval(int fn, const float* x, float* y, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = eval_cr(fn, x[i]);
@@ -126,6 +131,35 @@ int f32m_sweep(int fn, uint32_t lo, uint32_t hi, int threads, uint64_t* counts, 
         });
     }
     for (auto& th : pool) th.join();
+    return 0;
+}
+
+// pow over pairs: where 0 = the routine on the host, 1 = on the device,
+// 2 = f64 pow rounded once, 3 = the f32 libm powf.  Returns 0 or an error.
+int f32m_pow(int where, const float* x, const float* y, float* z, uint64_t n) {
+    if (where < 0 || where > 3 || !x || !y || !z) return -1;
+    if (where == 1) {
+        if (n == 0) return 0;
+        float *dx = nullptr, *dy = nullptr, *dz = nullptr;
+        hipError_t e = hipMalloc(&dx, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&dy, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&dz, n * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(dx, x, n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dy, y, n * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_pow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx, dy, dz, n);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(z, dz, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (dx) (void)hipFree(dx);
+        if (dy) (void)hipFree(dy);
+        if (dz) (void)hipFree(dz);
+        return (int)e;
+    }
+    for (uint64_t i = 0; i < n; ++i)
+        z[i] = where == 0 ? vr::f32m::pow_cr(x[i], y[i])
+               : where == 2 ? (float)pow((double)x[i], (double)y[i])
+                            : powf(x[i], y[i]);
     return 0;
 }
 

@@ -1,5 +1,6 @@
-"""The f32-only exp, ln, atan, sin and cos of the microfacet material
-(voidray_amd/csrc/f32math.h), evaluated on the host through the test probe.
+"""The f32-only exp, ln, atan, sin and cos of the microfacet material and
+the pow and exp of the tonemap pass (voidray_amd/csrc/f32math.h), evaluated
+on the host through the test probe.
 
 The reference's MicrofacetBSDF calls Rust's f32::exp, ln, atan, sin, cos
 (voidray_renderer/src/material/microfacet.rs:140-143, 183-186, 245-250 and the
@@ -174,3 +175,38 @@ def test_microfacet_sampling_chain():
         assert np.count_nonzero(th != F.evaluate("f64", "atan", a)) <= 1
         for fn in ("sin", "cos"):
             assert np.count_nonzero(F.evaluate("host", fn, th) != F.evaluate("f64", fn, th)) <= 1
+
+
+def test_pow_matches_f64_and_libm():
+    """pow_cr (the tonemap's GLSL pow: pow(c, 1/gamma), pow(2, exposure)) over
+    bit-uniform bases with the gammas and exponents the tonemap uses and over
+    random pairs: within one ulp of the f64 pow rounded once and of powf, and
+    equal to the f64 result on all but a few inputs."""
+    rng = np.random.default_rng(31)
+    lo, hi = F.bits(f32(1e-30)), F.bits(f32(1e6))
+    for y in (1 / 2.2, 1 / 1.8, 1 / 1.5, 2.2, 1.0, 0.5, 3.0, -1.5, 7.3):
+        x = rng.integers(lo, hi, 1 << 18, dtype=np.int64).astype(np.uint32).view(np.float32)
+        got = F.power("host", x, y)
+        d64 = F.ulp_distance(got, F.power("f64", x, y))
+        assert d64.max() <= 1 and np.count_nonzero(d64) <= 3, (y, np.count_nonzero(d64))
+        assert F.ulp_distance(got, F.power("libm", x, y)).max() <= 1
+    x = rng.integers(F.bits(f32(1e-3)), F.bits(f32(1e3)), 1 << 18, dtype=np.int64).astype(np.uint32).view(np.float32)
+    y = rng.uniform(-12, 12, 1 << 18).astype(np.float32)
+    got = F.power("host", x, y)
+    d64 = F.ulp_distance(got, F.power("f64", x, y))
+    assert d64.max() <= 1 and np.count_nonzero(d64) <= 3
+    # exposure: pow(2, e) for the exposures of the tonemap tests and demos
+    e = np.linspace(-8, 8, 4097, dtype=np.float32)
+    assert np.array_equal(F.power("host", np.full_like(e, 2.0), e), F.power("f64", np.full_like(e, 2.0), e))
+
+
+def test_pow_special_values_as_powf():
+    """Zeros, ones, infinities, NaN, negative bases with integer and
+    non-integer exponents, denormals: the same results as powf, sign of zero
+    and infinity included."""
+    sp = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, 3.0, -3.0, np.inf, -np.inf, np.nan,
+                   1e-40, -1e-40, 1e30, 2.0 ** 24, 2.0 ** 24 + 2, 0.3], np.float32)
+    X, Y = (a.ravel() for a in np.meshgrid(sp, sp))
+    got, want = F.power("host", X, Y), F.power("libm", X, Y)
+    same = (np.isnan(got) & np.isnan(want)) | ((got == want) & (np.signbit(got) == np.signbit(want)))
+    assert same.all(), [(float(a), float(b), float(c), float(d)) for a, b, c, d, k in zip(X, Y, got, want, same) if not k][:8]

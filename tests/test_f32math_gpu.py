@@ -46,3 +46,16 @@ def test_device_equals_host(fn):
     same = (dev.view(np.uint32) == host.view(np.uint32)) | (np.isnan(dev) & np.isnan(host))
     bad = np.flatnonzero(~same)
     assert bad.size == 0, [(float(x[i]).hex(), float(dev[i]).hex(), float(host[i]).hex()) for i in bad[:8]]
+
+
+def test_pow_device_equals_host():
+    rng = np.random.default_rng(41)
+    x = np.concatenate([rng.integers(F.bits(f32(1e-38)), F.bits(f32(1e6)), 1 << 20, dtype=np.int64)
+                        .astype(np.uint32).view(np.float32), np.array(SPECIAL, np.float32)])
+    y = np.concatenate([rng.choice(np.array([1 / 2.2, 1 / 1.8, 2.2, 1.0, 0.5, -1.5, 3.0], np.float32), 1 << 20),
+                        np.array(SPECIAL, np.float32)[::-1]])
+    announce(f"f32math: pow over {len(x)} pairs on the device")
+    dev, host = F.power("device", x, y), F.power("host", x, y)
+    same = (dev.view(np.uint32) == host.view(np.uint32)) | (np.isnan(dev) & np.isnan(host))
+    bad = np.flatnonzero(~same)
+    assert bad.size == 0, [(float(x[i]), float(y[i]), float(dev[i]), float(host[i])) for i in bad[:8]]

@@ -1,6 +1,7 @@
-// f32 exp, ln, sin, cos and atan computed in f32 arithmetic only, for the
-// MicrofacetBSDF material, whose reference calls Rust's f32::exp, ln, sin,
-// cos and atan (glibc's expf, logf, sinf, cosf, atanf on Linux).
+// f32 exp, ln, pow, sin, cos and atan computed in f32 arithmetic only, for
+// the MicrofacetBSDF material, whose reference calls Rust's f32::exp, ln, sin,
+// cos and atan (glibc's expf, logf, sinf, cosf, atanf on Linux), and for the
+// tonemap pass (GLSL pow and exp).
 //
 // Each function carries the value as a pair of floats (hi + lo, about 48 bits)
 // through range reduction, table lookup and polynomial, and rounds once at the
@@ -80,17 +81,16 @@ constexpr float kExp2Tab[32][2] = {
 // e^x.  x = (32n + j) ln2/32 + r with |r| <= ln2/64, where ln2/32 is split
 // into L1 + L2 + L3 (L1, L2 of 9 significant bits, so k·L1 and k·L2 are exact
 // for |k| < 2^15); e^x = 2^n · 2^(j/32) · e^r, e^r by its Taylor series to r^5.
-VR_HD float exp_cr(float x) {
-    if (!(x == x)) return x + x;
-    if (x > 89.0f) return __builtin_huge_valf();
-    if (x < -104.0f) return 0.0f;
+// e^(x + xlo) for x in [-104, 89] and |xlo| <= ulp(x)/2 (xlo = 0 for a
+// plain f32 argument; pow_cr passes the low word of y·ln x).
+VR_HD float exp_core(float x, float xlo) {
     constexpr float kInvL = 0x1.715476p+5f;  // 32/ln2
     constexpr float L1 = 0x1.63p-6f, L2 = -0x1.bdp-18f, L3 = -0x1.05c61p-34f;
     const float kf = rintf(x * kInvL);
     const int k = (int)kf;
     const float r1 = x - kf * L1;  // exact: kf·L1 within a factor 2 of x
     ff r = two_sum(r1, -(kf * L2));
-    r.lo = r.lo - kf * L3;
+    r.lo = (r.lo + xlo) - kf * L3;
     r = fast_two_sum(r.hi, r.lo);
     // e^r - 1 = r + r^2/2 + r^3 (1/6 + r/24 + r^2/120), r^2/2 kept exact
     const ff s = two_prod(r.hi, r.hi);
@@ -119,6 +119,13 @@ VR_HD float exp_cr(float x) {
     return q;
 }
 
+VR_HD float exp_cr(float x) {
+    if (!(x == x)) return x + x;
+    if (x > 89.0f) return __builtin_huge_valf();
+    if (x < -104.0f) return 0.0f;
+    return exp_core(x, 0.0f);
+}
+
 // ln(i/16), i = 11..23, as hi + lo.
 constexpr float kLnTab[13][2] = {
     {-0x1.7fafa4p-2f, 0x1.09fabap-28f},
@@ -141,9 +148,8 @@ constexpr float kLnTab[13][2] = {
 // |f| <= 1/45; 2 atanh f = 2f + (2/3) f^3 + f^5 (2/5 + (2/7) f^2), the first
 // two terms as hi + lo.  ln2 = LN2_1 + LN2_2 + LN2_3, the first two of 15
 // significant bits so that e·LN2_1 and e·LN2_2 are exact.
-VR_HD float ln_cr(float x) {
-    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
-    if (x == __builtin_huge_valf()) return x;
+// ln x as hi + lo (not renormalised) for finite x > 0.
+VR_HD ff ln_ff(float x) {
     int e = 0;
     if (x < 0x1p-126f) {
         x = x * 0x1p+24f;
@@ -175,12 +181,57 @@ VR_HD float ln_cr(float x) {
     const ff s1 = two_sum(ch, 2.0f * fh);
     const ff s2 = two_sum(s1.hi, a.hi);
     const float lo = (s1.lo + s2.lo) + (cl + (2.0f * fl + (a.lo + b)));
-    if (e == 0) return s2.hi + lo;
+    if (e == 0) return ff{s2.hi, lo};
     constexpr float LN2_1 = 0x1.62e4p-1f, LN2_2 = 0x1.7f7cp-20f, LN2_3 = 0x1.1cf79ap-36f;
     const float ef = (float)e;
     const ff s3 = two_sum(ef * LN2_1, s2.hi);
     const ff s4 = two_sum(s3.hi, ef * LN2_2);
-    return s4.hi + ((s3.lo + s4.lo) + (lo + ef * LN2_3));
+    return ff{s4.hi, (s3.lo + s4.lo) + (lo + ef * LN2_3)};
+}
+
+VR_HD float ln_cr(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    const ff r = ln_ff(x);
+    return r.hi + r.lo;
+}
+
+// x^y with C's powf special cases: pow(x, ±0) = 1 and pow(1, y) = 1 even for
+// NaN; a negative finite x gives NaN unless y is an integer, when the sign
+// follows y's parity; zeros and infinities as C99 Annex F.  Otherwise
+// e^(y·ln|x|) with y·ln|x| carried as hi + lo (about 2^-40 relative, so the
+// result is correctly rounded but within about 2^-40·|y ln x| of a midpoint).
+VR_HD float pow_cr(float x, float y) {
+    const float inf = __builtin_huge_valf();
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (!(x == x) || !(y == y)) return x + y;
+    const float ay = fabsf(y);
+    const bool yint = ay >= 0x1p+24f || truncf(y) == y;
+    const bool yodd = ay < 0x1p+24f && yint && (((int64_t)y) & 1) != 0;
+    if (ay == inf) {
+        const float ax = fabsf(x);
+        if (ax == 1.0f) return 1.0f;  // pow(-1, ±inf)
+        return (ax < 1.0f) == (y > 0.0f) ? 0.0f : inf;
+    }
+    const bool neg = (f2u(x) >> 31) != 0;  // -0 included
+    if (x == 0.0f) {
+        if (y < 0.0f) return yodd && neg ? -inf : inf;
+        return yodd && neg ? -0.0f : 0.0f;
+    }
+    if (fabsf(x) == inf) {
+        const float mag = y < 0.0f ? 0.0f : inf;
+        return yodd && neg ? -mag : mag;
+    }
+    if (neg && !yint) return __builtin_nanf("");
+    const ff l = ln_ff(fabsf(x));
+    ff t = two_prod(y, l.hi);
+    t.lo = t.lo + y * l.lo;
+    t = fast_two_sum(t.hi, t.lo);
+    float m;
+    if (t.hi > 89.0f) m = inf;
+    else if (t.hi < -104.0f) m = 0.0f;
+    else m = exp_core(t.hi, t.lo);
+    return neg && yodd ? -m : m;
 }
 
 // atan(j/16), j = 0..16, as hi + lo.

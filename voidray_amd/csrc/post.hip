@@ -4,6 +4,9 @@
 // clip_to_1024 reproduces the fixed [1024/8, 1024/8] dispatch of
 // render/post_process.rs:74 (pixels outside it are left untouched).  Modes from
 // VR_TONEMAP_LEGACY_NONE on are the legacy viewer's fragment pass instead.
+// pow and exp are the f32-only routines of f32math.h (the device library's
+// powf goes through f64 internally).
+#include "f32math.h"
 #include "render.h"
 
 namespace vr {
@@ -58,7 +61,7 @@ __device__ __forceinline__ void tm_luma_reinhard(float& r, float& g, float& b) {
     b *= k;
 }
 
-__device__ __forceinline__ float tm_rombindahouse1(float c) { return expf(-1.0f / (2.72f * c + 0.15f)); }
+__device__ __forceinline__ float tm_rombindahouse1(float c) { return f32m::exp_cr(-1.0f / (2.72f * c + 0.15f)); }
 
 // voidray/src/shaders/viewport_fragment.glsl:27-67: sampled = texel * scale,
 // colour = sampled.rgb * exposure, curve, pow(1 / gamma) (Filmic: gamma 1).
@@ -93,7 +96,7 @@ __device__ __forceinline__ f32x4 legacy_post(const vr_post_desc& p, f32x4 s) {
         default: break;
     }
     const float ig = 1.0f / gamma;
-    return f32x4{powf(r, ig), powf(g, ig), powf(b, ig), p.keep_alpha ? s.w * p.scale : 1.f};
+    return f32x4{f32m::pow_cr(r, ig), f32m::pow_cr(g, ig), f32m::pow_cr(b, ig), p.keep_alpha ? s.w * p.scale : 1.f};
 }
 
 __global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const f32x4* __restrict__ src, f32x4* __restrict__ dst) {
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const f32x4* __
         dst[i] = legacy_post(p, s);
         return;
     }
-    float ex = powf(2.f, p.exposure);
+    float ex = f32m::pow_cr(2.f, p.exposure);
     float r = (s.x * p.scale) * ex, g = (s.y * p.scale) * ex, b = (s.z * p.scale) * ex;
     switch (p.tonemap) {
         case VR_TONEMAP_ACES: tm_aces(r, g, b); break;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void k_tonemap(vr_post_desc p, const f32x4* __
         default: break;
     }
     float ig = 1.f / p.gamma;
-    dst[i] = f32x4{powf(r, ig), powf(g, ig), powf(b, ig), 1.f};
+    dst[i] = f32x4{f32m::pow_cr(r, ig), f32m::pow_cr(g, ig), f32m::pow_cr(b, ig), 1.f};
 }
 
 int tonemap_device(const vr_post_desc* desc, const float* d_src, float* d_dst, hipStream_t st) {
