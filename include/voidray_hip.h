@@ -380,7 +380,7 @@ typedef struct vr_tuning {
     uint32_t chunk_paths;     /* paths per wavefront chunk (queue length, at most 2^27);
                                  0 = 2^27, halved while the batch's path state would
                                  exceed 48 GiB */
-    uint32_t block;           /* workgroup size of raygen/shade/accumulate; 0 = 256 */
+    uint32_t block;           /* workgroup size of raygen/shade/accumulate; 0 = 1024 */
 } vr_tuning;
 
 #define VR_BATCH_GRAPH 1u          /* capture into a hipGraph and launch it (relaunchable) */

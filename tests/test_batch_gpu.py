@@ -160,7 +160,7 @@ def test_autotune_picks_a_candidate(gpu_ctx, scenes):
     br = BatchRenderer(gpu_ctx)
     announce("batch autotune")
     (chunk, block), ms = br.autotune(_batch(scenes, reqs, targets), reps=1)
-    assert chunk in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27) and block in (128, 256, 512)
+    assert chunk in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27) and block in (128, 256, 512, 1024)
     assert ms > 0.0
     # the tuned configuration still renders every request exactly
     want = _standalone(scenes, reqs)
