@@ -106,6 +106,29 @@ def test_default_boxes_beside_flat_boxes(gpu_ctx, oracle_lib):
         _compare(ref, acc.hit(o, d, engine=engine), f"default-boxes/{engine}")
 
 
+def test_trace_timing_only_on_request(gpu_ctx):
+    """VR_TRACE_TIMING: vr_trace_last_timing reports the wide kernel's device
+    time only after a trace that asked for it, and timing changes no hit."""
+    import torch
+    scene = W.random_triangle_scene(4096, seed=9)
+    o, d = W.microbench_rays(8192, res=64)
+    acc = scene.build_acceleration(gpu_ctx)
+    announce("trace timing on request")
+    to, td = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+    outs = []
+    for timing in (False, True):
+        prim = torch.empty(len(o), dtype=torch.int32, device="cuda")
+        acc.trace_device(to.data_ptr(), td.data_ptr(), len(o), d_prim=prim.data_ptr(), timing=timing)
+        torch.cuda.synchronize()
+        if timing:
+            assert acc.last_wide_ms() > 0.0
+        else:
+            with pytest.raises(N.VoidrayError):
+                acc.last_wide_ms()
+        outs.append(prim.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_empty_scene_and_zero_rays(gpu_ctx):
     scene = Scene.empty()
     announce("trace empty scene")
