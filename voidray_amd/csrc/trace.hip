@@ -9,10 +9,11 @@
 namespace vr {
 
 constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
-// LDS stack entries per lane (8 B each): 5 KB per wave, so the 70 VGPRs, not
-// LDS, bound the resident waves (7 per SIMD).  Measured against 12 entries (6
-// waves): microbench 0.8% faster, terrain frame 2.4% faster, the cornell room
-// (long bounce chains, deeper stacks) 5% slower; 14 entries 4% slower.
+// LDS stack entries per lane (8 B each): 5 KB per wave, so 32 waves per CU
+// (160 KB of LDS) fit, as many as the closest-hit kernel's 64 VGPRs allow
+// (8 per SIMD).  Measured against 12 entries (6 waves): microbench 0.8%
+// faster, terrain frame 2.4% faster, the cornell room (long bounce chains,
+// deeper stacks) 5% slower; 14 entries 4% slower.
 constexpr int kLdsStack = 10;
 constexpr int kLdsStackAny = 12;  // any hit: 6 and 10 entries measured 3.8% and 0.4% slower than 12, 8 2.4%
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
@@ -197,10 +198,10 @@ __device__ __forceinline__ void trace_persist(const TraceArgs& a) {
     }
 }
 
-// Closest hit: registers held to what 8 waves per SIMD allow (the 6 KB LDS
-// stack then bounds residency at 6 waves; a 4 KB stack gave 8 waves and ~1%
-// more on short rays but spilled more on long ones).  Any hit keeps the
-// registers it wants (7 waves): held to 64 it spills.
+// Closest hit: registers held to what 8 waves per SIMD allow (64 VGPRs, no
+// scratch; the 5 KB LDS ring allows 8 as well).  Any hit keeps the registers
+// it wants (72 VGPRs, 7 waves; its 6 KB ring allows 6.7 per SIMD): held to 64
+// it spills.
 template <bool ANY, bool WT = false, bool INST = false>
 __global__ void k_trace_persist(TraceArgs a);
 template <>
