@@ -15,7 +15,7 @@ constexpr int kWave = 64;        // gfx950 wavefront size; one wave per block
 // faster, terrain frame 2.4% faster, the cornell room (long bounce chains,
 // deeper stacks) 5% slower; 14 entries 4% slower.
 constexpr int kLdsStack = 10;
-constexpr int kLdsStackAny = 12;  // any hit: 6 and 10 entries measured 3.8% and 0.4% slower than 12, 8 2.4%
+constexpr int kLdsStackAny = 12;  // any hit: 6, 8, 10 and 11 entries each measured slower (0.4-3.8%)
 constexpr int kSpillCap = 48;    // global spill entries per lane beyond the LDS part
 constexpr int kRefill = 40;      // idle lanes that trigger a wave-wide refill (closest hit; 24, 32, 48 and 56 measured slower)
 constexpr int kRefillAny = 40;   // the same for any-hit queries (shorter rays: refill less often)
