@@ -154,7 +154,7 @@ def scene_digest(scene):
 
 
 def generator_cases():
-    """The procedural workloads: configs 1-4 at full size and the scenes the tests use."""
+    """The procedural workloads: every benchmark workload at full size and the scenes the tests use."""
     return {
         "cornell_512": lambda: W.cornell_room(),
         "terrain_1080p": lambda: W.terrain_spheres(),

@@ -259,8 +259,8 @@ def test_trained_cache_gains_over_the_plain_tracer(gpu_ctx):
     assert m["trained"][2] < m["plain"][2]
 
 
-def test_cache_at_the_config_sizes_gains_at_equal_time(gpu_ctx):
-    """The radiance-cache leg at config 5's sizes (bench.py's own routine):
+def test_cache_at_the_benchmark_sizes_gains_at_equal_time(gpu_ctx):
+    """The radiance-cache leg at the benchmark's sizes (bench.py's own routine):
     2^19-entry levels, 2^16 training paths per frame, the 2.9 M-triangle
     terrain at 960x540 against the plain tracer at 4096 spp, queries at the
     first rough vertex from bounce 2.  At the cached run's final time the

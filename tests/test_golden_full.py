@@ -26,7 +26,7 @@ def test_full_fixture_inputs_and_sample_blocks():
 
 
 def test_procedural_generators_reproduce_their_digests():
-    """Every procedural workload (configs 1-4 at full size: the cornell room,
+    """Every procedural workload (the benchmark workloads at full size: the cornell room,
     the 1080p terrain with 200 spheres, textures and HDR sky, the 8-scene
     serving pool and its 64 requests; and the scenes the tests render) gives
     the committed SHA-256 of everything it hands to the library."""

@@ -7,7 +7,7 @@ reference would be given) and the FP64 oracle of that flattening.
 * traces: the same prim as the flattened scene for >= 99.8% of rays (the
   rest lie within rounding of a shared edge), closest and any hit, every
   engine, on the rotated / scaled / mirrored instance case and on the
-  terrain with 200 instanced spheres (config 3's scene, reduced terrain);
+  terrain with 200 instanced spheres (the 1080p benchmark scene, reduced terrain);
 * renders (extended estimator, same random streams): the instanced image is
   within the f32-vs-f64 rounding floor of the FP64 oracle of the flattened
   scene and far below Monte Carlo noise; normals of mirrored instances
